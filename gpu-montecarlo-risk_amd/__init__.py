@@ -1,0 +1,339 @@
+"""MI355X (gfx950) Monte Carlo crash-risk model: Python view of the C ABI.
+
+The product is ``_lib/libmcr_hip.so`` (built from ``csrc/`` by ``build()``), whose
+C ABI is declared in ``include/mcr_hip.h``.  This module only binds it with
+ctypes; there is no Python or CPU fallback, and every entry point raises when
+the library is missing or a call fails.
+
+Reference API mirrored here (same names, argument meaning and return values):
+  monte_carlo_crash_simulation(x, N, k, mu, Sigma, M)  -> (P_hat, count, S_values)
+      src/01-python-serial/python_serial.py:11-75
+  compute_accuracy(P_hat, M) -> (std_error, ci_lower, ci_upper)
+      src/01-python-serial/python_serial.py:77-105
+  get_hip_model() -> ModelFunctions {name, run_model}
+      src/model_interface.h:33-36, src/05-GPU/monte_carlos_cuda.cu:165-171
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+import re
+import subprocess
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libmcr_hip.so")
+HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "mcr_hip.h")
+
+# constants from include/mcr_hip.h
+MCR_OK = 0
+MCR_E_ARG, MCR_E_UNSUPPORTED, MCR_E_DEVICE, MCR_E_NOMEM, MCR_E_FACTOR = -1, -2, -3, -4, -5
+MCR_MAX_ALPHA = 4
+MCR_MAX_N = 608
+PREC_FP32, PREC_BF16X3, PREC_AUTO = 0, 1, 2
+KERNEL_AUTO, KERNEL_VALU_REG, KERNEL_VALU_TILE, KERNEL_MFMA = 0, 1, 2, 3
+F_VAR, F_NEAR, F_SHIST, F_TEST_NARROW = 0x1, 0x2, 0x4, 0x100
+KERNEL_NAMES = {KERNEL_VALU_REG: "valu_reg", KERNEL_VALU_TILE: "valu_tile", KERNEL_MFMA: "mfma"}
+
+
+class McrError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"mcr error {code}: {msg}")
+        self.code = code
+
+
+def build(jobs: int = 8) -> str:
+    """Compile csrc/ for gfx950 into _lib/libmcr_hip.so (hipcc; no GPU needed)."""
+    subprocess.run(["make", "-s", f"-j{jobs}", "-C", os.path.join(_HERE, "csrc")], check=True)
+    return LIB_PATH
+
+
+_f64p = ctypes.POINTER(ctypes.c_double)
+_f32p = ctypes.POINTER(ctypes.c_float)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+
+
+class Query(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("mu", _f64p), ("sigma", _f64p), ("ld_sigma", ctypes.c_int),
+                ("weights", _f64p), ("x", ctypes.c_double), ("k", ctypes.c_int),
+                ("paths", ctypes.c_int64), ("seed", ctypes.c_uint64),
+                ("path_offset", ctypes.c_uint64), ("precision", ctypes.c_int),
+                ("kernel", ctypes.c_int), ("flags", ctypes.c_int), ("n_alpha", ctypes.c_int),
+                ("alpha", ctypes.c_double * MCR_MAX_ALPHA), ("S_out", _i32p),
+                ("loss_out", _f32p), ("s_hist", _i64p)]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("count", ctypes.c_int64), ("near_threshold", ctypes.c_int64),
+                ("p_hat", ctypes.c_double), ("std_error", ctypes.c_double),
+                ("ci_lower", ctypes.c_double), ("ci_upper", ctypes.c_double),
+                ("var", ctypes.c_double * MCR_MAX_ALPHA), ("es", ctypes.c_double * MCR_MAX_ALPHA),
+                ("tail_sum_fixed", ctypes.c_int64 * MCR_MAX_ALPHA), ("var_passes", ctypes.c_int),
+                ("loss_mean", ctypes.c_double), ("loss_sd", ctypes.c_double),
+                ("jitter", ctypes.c_double), ("factor_attempts", ctypes.c_int),
+                ("factor_clamped", ctypes.c_int), ("kernel", ctypes.c_int),
+                ("ms_kernel", ctypes.c_double), ("ms_total", ctypes.c_double)]
+
+
+class FactorInfo(ctypes.Structure):
+    _fields_ = [("attempts", ctypes.c_int), ("jitter", ctypes.c_double), ("clamped", ctypes.c_int)]
+
+
+class ModelExport(ctypes.Structure):
+    _fields_ = [("L64", _f64p), ("L32", _f32p), ("mu32", _f32p), ("v32", _f32p),
+                ("mp32", ctypes.c_float), ("thr32", ctypes.c_float), ("loss_sd", ctypes.c_double)]
+
+
+# GSL-layout structs of include/model_interface.h (for the plug-in boundary)
+class GslVector(ctypes.Structure):
+    _fields_ = [("size", ctypes.c_size_t), ("stride", ctypes.c_size_t), ("data", _f64p),
+                ("block", ctypes.c_void_p), ("owner", ctypes.c_int)]
+
+
+class GslMatrix(ctypes.Structure):
+    _fields_ = [("size1", ctypes.c_size_t), ("size2", ctypes.c_size_t), ("tda", ctypes.c_size_t),
+                ("data", _f64p), ("block", ctypes.c_void_p), ("owner", ctypes.c_int)]
+
+
+class MonteCarloResult(ctypes.Structure):
+    _fields_ = [("P_hat", ctypes.c_double), ("count", ctypes.c_int), ("S_values", _i32p),
+                ("std_error", ctypes.c_double), ("ci_lower", ctypes.c_double),
+                ("ci_upper", ctypes.c_double)]
+
+
+class MonteCarloParams(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_double), ("N", ctypes.c_int), ("k", ctypes.c_int),
+                ("mu", ctypes.POINTER(GslVector)), ("Sigma", ctypes.POINTER(GslMatrix)),
+                ("M", ctypes.c_int), ("random_seed", ctypes.c_ulong)]
+
+
+ModelRunFunc = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.POINTER(MonteCarloParams),
+                                ctypes.POINTER(MonteCarloResult))
+
+
+class ModelFunctions(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char_p), ("run_model", ModelRunFunc)]
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """The loaded product library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"HIP library not built: {LIB_PATH} (run build())")
+        L = ctypes.CDLL(LIB_PATH)
+        L.mcr_init.argtypes = [ctypes.c_int]
+        L.mcr_last_error.restype = ctypes.c_char_p
+        L.mcr_run.argtypes = [ctypes.POINTER(Query), ctypes.POINTER(Result)]
+        L.mcr_factor.argtypes = [ctypes.c_int, _f64p, ctypes.c_int, _f64p,
+                                 ctypes.POINTER(FactorInfo)]
+        L.mcr_prepare.argtypes = [ctypes.POINTER(Query), ctypes.POINTER(ModelExport)]
+        L.mcr_normals.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int64, ctypes.c_int,
+                                  _f32p]
+        L.mcr_synth_factor_model.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                             ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                             ctypes.c_uint64, _f64p, _f64p, _f64p]
+        L.hip_simulate.argtypes = [ctypes.POINTER(MonteCarloParams),
+                                   ctypes.POINTER(MonteCarloResult)]
+        L.get_hip_model.restype = ModelFunctions
+        _lib = L
+    return _lib
+
+
+def declared_functions() -> list[str]:
+    """Function names declared in include/mcr_hip.h (the exported C ABI)."""
+    text = open(HEADER_PATH).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return re.findall(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_]*\s*\*?\s*([a-z_][a-z0-9_]*)\s*\(",
+                      text, flags=re.M)
+
+
+def _check(code: int) -> None:
+    if code != MCR_OK:
+        raise McrError(code, lib().mcr_last_error().decode())
+
+
+def _ptr(a, t):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+@dataclass
+class RiskResult:
+    count: int
+    near_threshold: int
+    p_hat: float
+    std_error: float
+    ci_lower: float
+    ci_upper: float
+    var: list = field(default_factory=list)
+    es: list = field(default_factory=list)
+    tail_sum_fixed: list = field(default_factory=list)
+    var_passes: int = 0
+    loss_mean: float = 0.0
+    loss_sd: float = 0.0
+    jitter: float = 0.0
+    factor_attempts: int = 0
+    factor_clamped: int = 0
+    kernel: str = ""
+    ms_kernel: float = 0.0
+    ms_total: float = 0.0
+    S: np.ndarray | None = None
+    loss: np.ndarray | None = None
+    s_hist: np.ndarray | None = None
+
+
+def run(mu, sigma, x: float, k: int, paths: int, seed: int = 0, *, weights=None,
+        precision: int = PREC_FP32, kernel: int = KERNEL_AUTO, alphas=(), near: bool = False,
+        want_S: bool = False, want_loss: bool = False, want_hist: bool = False,
+        path_offset: int = 0, extra_flags: int = 0) -> RiskResult:
+    """One risk query on the GPU (mcr_run)."""
+    mu = np.ascontiguousarray(mu, dtype=np.float64)
+    sigma = np.ascontiguousarray(sigma, dtype=np.float64)
+    n = int(mu.shape[0])
+    w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+    S = np.zeros(paths, dtype=np.int32) if want_S else None
+    loss = np.zeros(paths, dtype=np.float32) if want_loss else None
+    hist = np.zeros(n + 1, dtype=np.int64) if want_hist else None
+    flags = extra_flags
+    if alphas:
+        flags |= F_VAR
+    if near:
+        flags |= F_NEAR
+    if want_hist:
+        flags |= F_SHIST
+    q = Query()
+    q.n = n
+    q.mu = _ptr(mu, _f64p)
+    q.sigma = _ptr(sigma, _f64p)
+    q.ld_sigma = int(sigma.shape[1]) if sigma.ndim == 2 else n
+    q.weights = _ptr(w, _f64p)
+    q.x = float(x)
+    q.k = int(k)
+    q.paths = int(paths)
+    q.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
+    q.path_offset = int(path_offset)
+    q.precision = int(precision)
+    q.kernel = int(kernel)
+    q.flags = flags
+    q.n_alpha = len(alphas)
+    for i, a in enumerate(alphas):
+        q.alpha[i] = float(a)
+    q.S_out = _ptr(S, _i32p)
+    q.loss_out = _ptr(loss, _f32p)
+    q.s_hist = _ptr(hist, _i64p)
+    r = Result()
+    _check(lib().mcr_run(ctypes.byref(q), ctypes.byref(r)))
+    na = len(alphas)
+    return RiskResult(count=r.count, near_threshold=r.near_threshold, p_hat=r.p_hat,
+                      std_error=r.std_error, ci_lower=r.ci_lower, ci_upper=r.ci_upper,
+                      var=list(r.var)[:na], es=list(r.es)[:na],
+                      tail_sum_fixed=list(r.tail_sum_fixed)[:na], var_passes=r.var_passes,
+                      loss_mean=r.loss_mean, loss_sd=r.loss_sd, jitter=r.jitter,
+                      factor_attempts=r.factor_attempts, factor_clamped=r.factor_clamped,
+                      kernel=KERNEL_NAMES.get(r.kernel, str(r.kernel)), ms_kernel=r.ms_kernel,
+                      ms_total=r.ms_total, S=S, loss=loss, s_hist=hist)
+
+
+def factor(sigma):
+    """GPU FP64 Cholesky with the jitter policy: (L, attempts, jitter, clamped)."""
+    sigma = np.ascontiguousarray(sigma, dtype=np.float64)
+    n = sigma.shape[0]
+    L = np.zeros((n, n), dtype=np.float64)
+    info = FactorInfo()
+    _check(lib().mcr_factor(n, _ptr(sigma, _f64p), n, _ptr(L, _f64p), ctypes.byref(info)))
+    return L, info.attempts, info.jitter, info.clamped
+
+
+def prepare(mu, sigma, x: float, weights=None) -> dict:
+    """The fp32 model the path kernels read (factor computed on the GPU)."""
+    mu = np.ascontiguousarray(mu, dtype=np.float64)
+    sigma = np.ascontiguousarray(sigma, dtype=np.float64)
+    n = mu.shape[0]
+    w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+    q = Query()
+    q.n, q.mu, q.sigma, q.ld_sigma = n, _ptr(mu, _f64p), _ptr(sigma, _f64p), n
+    q.weights, q.x, q.k, q.paths = _ptr(w, _f64p), float(x), 1, 1
+    q.precision, q.kernel = PREC_FP32, KERNEL_VALU_TILE
+    L64 = np.zeros((n, n), dtype=np.float64)
+    L32 = np.zeros((n, n), dtype=np.float32)
+    mu32 = np.zeros(n, dtype=np.float32)
+    v32 = np.zeros(n, dtype=np.float32)
+    ex = ModelExport()
+    ex.L64, ex.L32 = _ptr(L64, _f64p), _ptr(L32, _f32p)
+    ex.mu32, ex.v32 = _ptr(mu32, _f32p), _ptr(v32, _f32p)
+    _check(lib().mcr_prepare(ctypes.byref(q), ctypes.byref(ex)))
+    return {"L64": L64, "L32": L32, "mu32": mu32, "v32": v32, "mp32": np.float32(ex.mp32),
+            "thr32": np.float32(ex.thr32), "loss_sd": ex.loss_sd}
+
+
+def normals(seed: int, path0: int, npaths: int, n: int) -> np.ndarray:
+    """The device normal stream, z[p, j] for paths path0 .. path0+npaths-1."""
+    out = np.zeros((npaths, n), dtype=np.float32)
+    _check(lib().mcr_normals(int(seed) & 0xFFFFFFFFFFFFFFFF, int(path0), int(npaths), int(n),
+                             _ptr(out, _f32p)))
+    return out
+
+
+def synth_factor_model(n: int, K: int, T: int, seed: int, drift: float = 2e-4,
+                       fvol: float = 0.01, ivol: float = 0.01, want_returns: bool = False):
+    """Seeded factor-model returns -> (mu, Sigma[, returns]) (host computation)."""
+    mu = np.zeros(n, dtype=np.float64)
+    sigma = np.zeros((n, n), dtype=np.float64)
+    R = np.zeros((T, n), dtype=np.float64) if want_returns else None
+    _check(lib().mcr_synth_factor_model(n, K, T, drift, fvol, ivol, int(seed), _ptr(mu, _f64p),
+                                        _ptr(sigma, _f64p), _ptr(R, _f64p)))
+    return (mu, sigma, R) if want_returns else (mu, sigma)
+
+
+# ---------------------------------------------------------------------------
+# Mirror of the reference's Python prototype (python_serial.py) on the GPU.
+# ---------------------------------------------------------------------------
+def monte_carlo_crash_simulation(x, N, k, mu, Sigma, M, seed: int = 0,
+                                 precision: int = PREC_FP32):
+    """P(at least k of N assets fall below -x), as python_serial.py:11-75.
+
+    Returns (P_hat, count, S_values) with S_values the per-path crash counts.
+    """
+    mu = np.asarray(mu, dtype=np.float64)
+    if mu.shape[0] != N:
+        raise ValueError(f"N = {N} does not match mu ({mu.shape[0]})")
+    r = run(mu, Sigma, x, k, M, seed, precision=precision, want_S=True)
+    return r.p_hat, r.count, r.S
+
+
+def compute_accuracy(P_hat, M):
+    """Standard error and clamped 95% CI, as python_serial.py:77-105."""
+    std_error = math.sqrt(P_hat * (1 - P_hat) / M)
+    margin = 1.96 * std_error
+    return std_error, max(0, P_hat - margin), min(1, P_hat + margin)
+
+
+def get_hip_model() -> ModelFunctions:
+    """The plug-in model record {name: "hip", run_model: hip_simulate}."""
+    return lib().get_hip_model()
+
+
+def make_params(mu, Sigma, x: float, k: int, M: int, seed: int, *, mu_stride: int = 1,
+                sigma_tda: int | None = None):
+    """MonteCarloParams with GSL-layout views (keep the returned tuple alive)."""
+    mu = np.asarray(mu, dtype=np.float64)
+    Sigma = np.asarray(Sigma, dtype=np.float64)
+    n = mu.shape[0]
+    mbuf = np.zeros(n * mu_stride, dtype=np.float64)
+    mbuf[::mu_stride] = mu
+    tda = sigma_tda or n
+    sbuf = np.zeros(n * tda, dtype=np.float64)
+    for i in range(n):
+        sbuf[i * tda:i * tda + n] = Sigma[i]
+    gv = GslVector(n, mu_stride, mbuf.ctypes.data_as(_f64p), None, 0)
+    gm = GslMatrix(n, n, tda, sbuf.ctypes.data_as(_f64p), None, 0)
+    p = MonteCarloParams(float(x), n, int(k), ctypes.pointer(gv), ctypes.pointer(gm), int(M),
+                         int(seed))
+    return p, (mbuf, sbuf, gv, gm)

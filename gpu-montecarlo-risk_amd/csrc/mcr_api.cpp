@@ -1,0 +1,806 @@
+// mcr_api.cpp -- C ABI of the HIP model (include/mcr_hip.h).
+//
+// One query (mcr_run) replaces the body of cuda_simulate
+// (src/05-GPU/monte_carlos_cuda.cu:75-163): inputs go to the device once, the
+// factor and every layout are built there (k_factor, k_prep), the fused path
+// kernel reduces on device and only a few hundred bytes come back, plus the
+// handful of losses that bracket each requested quantile.  Statistics follow
+// src/05-GPU/monte_carlos_cuda.cu:146-150 (P_hat, SE, clamped 95% CI).
+//
+// Device memory is owned by the library: one grow-only arena per process and a
+// pinned staging buffer, reused across queries, released by mcr_shutdown().
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "../../include/mcr_hip.h"
+#include "mcr_internal.h"
+
+using namespace mcr;
+
+namespace {
+
+thread_local char g_err[512] = "";
+
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+#define HIP_TRY(expr)                                                              \
+  do {                                                                             \
+    hipError_t e_ = (expr);                                                        \
+    if (e_ != hipSuccess) {                                                        \
+      set_error("%s failed: %s (line %d)", #expr, hipGetErrorString(e_), __LINE__); \
+      return MCR_E_DEVICE;                                                         \
+    }                                                                              \
+  } while (0)
+
+#define MCR_TRY(expr)     \
+  do {                    \
+    int r_ = (expr);      \
+    if (r_ != MCR_OK) return r_; \
+  } while (0)
+
+constexpr size_t kAlign = 256;
+constexpr uint64_t kLaunchPaths = 1ull << 32;   // paths per launch (keeps lane tallies 32-bit)
+constexpr uint64_t kPerPathChunk = 1ull << 24;  // per-path outputs staged per launch
+
+size_t align_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
+
+struct Context {
+  bool ready = false;
+  int device = 0;
+  int num_cus = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  void* arena = nullptr;
+  size_t arena_bytes = 0;
+  void* host = nullptr;  // pinned staging
+  size_t host_bytes = 0;
+};
+
+Context g_ctx;
+std::mutex g_mu;
+
+int ensure_init(int device) {
+  if (g_ctx.ready) return MCR_OK;
+  int count = 0;
+  HIP_TRY(hipGetDeviceCount(&count));
+  if (count <= 0) {
+    set_error("no HIP device");
+    return MCR_E_DEVICE;
+  }
+  if (device < 0 || device >= count) {
+    set_error("device %d out of range (%d devices)", device, count);
+    return MCR_E_ARG;
+  }
+  HIP_TRY(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, device));
+  g_ctx.device = device;
+  g_ctx.num_cus = prop.multiProcessorCount;
+  HIP_TRY(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
+  HIP_TRY(hipEventCreate(&g_ctx.ev0));
+  HIP_TRY(hipEventCreate(&g_ctx.ev1));
+  g_ctx.ready = true;
+  return MCR_OK;
+}
+
+int reserve_arena(size_t bytes) {
+  if (bytes <= g_ctx.arena_bytes) return MCR_OK;
+  if (g_ctx.arena) {
+    HIP_TRY(hipStreamSynchronize(g_ctx.stream));
+    HIP_TRY(hipFree(g_ctx.arena));
+    g_ctx.arena = nullptr;
+    g_ctx.arena_bytes = 0;
+  }
+  size_t want = std::max(bytes, (size_t)(g_ctx.arena_bytes * 3 / 2));
+  if (hipMalloc(&g_ctx.arena, want) != hipSuccess) {
+    (void)hipGetLastError();
+    set_error("device allocation of %zu bytes failed", want);
+    return MCR_E_NOMEM;
+  }
+  g_ctx.arena_bytes = want;
+  return MCR_OK;
+}
+
+int reserve_host(size_t bytes) {
+  if (bytes <= g_ctx.host_bytes) return MCR_OK;
+  if (g_ctx.host) {
+    HIP_TRY(hipStreamSynchronize(g_ctx.stream));
+    HIP_TRY(hipHostFree(g_ctx.host));
+    g_ctx.host = nullptr;
+    g_ctx.host_bytes = 0;
+  }
+  size_t want = std::max(bytes, (size_t)(g_ctx.host_bytes * 3 / 2));
+  if (hipHostMalloc(&g_ctx.host, want, hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    set_error("pinned host allocation of %zu bytes failed", want);
+    return MCR_E_NOMEM;
+  }
+  g_ctx.host_bytes = want;
+  return MCR_OK;
+}
+
+// Carves 256-byte aligned pieces out of a base pointer (device or host).
+struct Carver {
+  char* base;
+  size_t off = 0;
+  template <typename T>
+  T* take(size_t count) {
+    T* p = base ? reinterpret_cast<T*>(base + off) : nullptr;
+    off += align_up(count * sizeof(T));
+    return p;
+  }
+};
+
+// Inverse standard normal cdf by Newton on erfc from a bisection bracket.
+double inv_norm_cdf(double p) {
+  double lo = -40.0, hi = 40.0;
+  for (int i = 0; i < 200; ++i) {
+    const double mid = 0.5 * (lo + hi);
+    if (0.5 * std::erfc(-mid / std::sqrt(2.0)) < p) lo = mid; else hi = mid;
+    if (hi - lo < 1e-13) break;
+  }
+  double x = 0.5 * (lo + hi);
+  for (int i = 0; i < 3; ++i) {
+    const double f = 0.5 * std::erfc(-x / std::sqrt(2.0)) - p;
+    const double d = std::exp(-0.5 * x * x) / std::sqrt(2.0 * M_PI);
+    if (d <= 0.0) break;
+    x -= f / d;
+  }
+  return x;
+}
+
+// Exact ceil(alpha * M) using the binary value of alpha.
+int64_t rank_of(double alpha, int64_t M) {
+  int e = 0;
+  const double m = std::frexp(alpha, &e);           // alpha = m 2^e, m in [0.5, 1)
+  const unsigned __int128 mant = (unsigned __int128)std::ldexp(m, 53);  // exact 53-bit
+  const int sh = 53 - e;                             // alpha = mant / 2^sh, sh >= 53
+  const unsigned __int128 prod = mant * (unsigned __int128)M;
+  unsigned __int128 q = sh >= 128 ? 0 : (prod >> sh);
+  const unsigned __int128 rem = sh >= 128 ? prod : (prod - (q << sh));
+  if (rem != 0) q += 1;
+  int64_t r = (int64_t)q;
+  if (r < 1) r = 1;
+  if (r > M) r = M;
+  return r;
+}
+
+struct DevBuffers {
+  double *A, *mu, *w, *L, *v64;
+  DevInfo* info;
+  DevAccum* acc;
+  float *colpack, *L32, *mu32, *v32, *rowinit;
+  uint4* lfrag;
+  float* gather;
+  unsigned* hist;
+  unsigned long long* s_hist;
+  int32_t* S_out;
+  float* loss_out;
+};
+
+struct Plan {
+  int n, T, S, r8;
+  int kernel;
+  int n_alpha;
+  bool per_path;
+  size_t chunk;  // per-path chunk
+};
+
+size_t layout(const Plan& p, char* base, DevBuffers* b) {
+  Carver c{base};
+  const size_t n = (size_t)p.n;
+  b->A = c.take<double>(n * n);
+  b->mu = c.take<double>(n);
+  b->w = c.take<double>(n);
+  b->L = c.take<double>(n * n);
+  b->v64 = c.take<double>(n);
+  b->info = c.take<DevInfo>(1);
+  b->acc = c.take<DevAccum>(1);
+  b->colpack = c.take<float>(colpack_size(p.n));
+  b->L32 = c.take<float>((size_t)p.r8 * p.r8);
+  b->mu32 = c.take<float>(n);
+  b->v32 = c.take<float>(n);
+  b->lfrag = c.take<uint4>((size_t)p.T * p.S * 64 * 2);
+  b->rowinit = c.take<float>((size_t)32 * p.T);
+  b->gather = c.take<float>((size_t)std::max(p.n_alpha, 1) * kGatherCap);
+  b->hist = c.take<unsigned>(kHistBins);
+  b->s_hist = c.take<unsigned long long>(n + 1);
+  b->S_out = p.per_path ? c.take<int32_t>(p.chunk) : nullptr;
+  b->loss_out = p.per_path ? c.take<float>(p.chunk) : nullptr;
+  return c.off;
+}
+
+int choose_kernel(const mcr_query* q, Plan* p) {
+  const int n = q->n;
+  int k = q->kernel;
+  if (k == MCR_KERNEL_AUTO) {
+    if (q->precision == MCR_PREC_FP32) {
+      k = n <= kRegMaxN ? MCR_KERNEL_VALU_REG : MCR_KERNEL_VALU_TILE;
+    } else if (q->precision == MCR_PREC_BF16X3) {
+      if (n > 32 * kMfmaMaxT - 1) {
+        set_error("bf16x3 MFMA kernel supports n <= %d (n = %d)", 32 * kMfmaMaxT - 1, n);
+        return MCR_E_UNSUPPORTED;
+      }
+      k = MCR_KERNEL_MFMA;
+    } else if (q->precision == MCR_PREC_AUTO) {
+      k = n <= 32 * kMfmaMaxT - 1 ? MCR_KERNEL_MFMA
+                                  : (n <= kRegMaxN ? MCR_KERNEL_VALU_REG : MCR_KERNEL_VALU_TILE);
+    } else {
+      set_error("unknown precision %d", q->precision);
+      return MCR_E_ARG;
+    }
+  }
+  if (k == MCR_KERNEL_VALU_REG && n > kRegMaxN) {
+    set_error("register kernel supports n <= %d (n = %d)", kRegMaxN, n);
+    return MCR_E_UNSUPPORTED;
+  }
+  if (k == MCR_KERNEL_MFMA && n > 32 * kMfmaMaxT - 1) {
+    set_error("MFMA kernel supports n <= %d (n = %d)", 32 * kMfmaMaxT - 1, n);
+    return MCR_E_UNSUPPORTED;
+  }
+  if (k < MCR_KERNEL_VALU_REG || k > MCR_KERNEL_MFMA) {
+    set_error("unknown kernel %d", k);
+    return MCR_E_ARG;
+  }
+  p->kernel = k;
+  if (k == MCR_KERNEL_MFMA) {
+    p->T = (n + 1 + 31) / 32;
+    p->S = (n + 15) / 16;
+  } else {
+    p->T = 0;
+    p->S = 0;
+  }
+  return MCR_OK;
+}
+
+int validate(const mcr_query* q) {
+  if (!q) {
+    set_error("query is NULL");
+    return MCR_E_ARG;
+  }
+  if (q->n < 1 || q->n > MCR_MAX_N) {
+    set_error("n = %d outside [1, %d]", q->n, MCR_MAX_N);
+    return q->n > MCR_MAX_N ? MCR_E_UNSUPPORTED : MCR_E_ARG;
+  }
+  if (!q->mu || !q->sigma) {
+    set_error("mu and sigma are required");
+    return MCR_E_ARG;
+  }
+  if (q->ld_sigma < q->n) {
+    set_error("ld_sigma = %d < n = %d", q->ld_sigma, q->n);
+    return MCR_E_ARG;
+  }
+  if (q->paths < 1) {
+    set_error("paths = %lld must be >= 1", (long long)q->paths);
+    return MCR_E_ARG;
+  }
+  if (!std::isfinite(q->x)) {
+    set_error("x is not finite");
+    return MCR_E_ARG;
+  }
+  if ((q->flags & MCR_F_VAR) && (q->n_alpha < 1 || q->n_alpha > MCR_MAX_ALPHA)) {
+    set_error("n_alpha = %d outside [1, %d]", q->n_alpha, MCR_MAX_ALPHA);
+    return MCR_E_ARG;
+  }
+  if (q->flags & MCR_F_VAR) {
+    for (int i = 0; i < q->n_alpha; ++i)
+      if (!(q->alpha[i] > 0.0 && q->alpha[i] < 1.0)) {
+        set_error("alpha[%d] = %g outside (0, 1)", i, q->alpha[i]);
+        return MCR_E_ARG;
+      }
+  }
+  if ((q->flags & MCR_F_SHIST) && !q->s_hist) {
+    set_error("MCR_F_SHIST needs s_hist");
+    return MCR_E_ARG;
+  }
+  if (q->path_offset + (uint64_t)q->paths < q->path_offset) {
+    set_error("path ids overflow 64 bits");
+    return MCR_E_ARG;
+  }
+  return MCR_OK;
+}
+
+struct Staged {
+  Plan plan;
+  DevBuffers d;
+  char* host;
+};
+
+// Uploads inputs, factors and prepares on the device.  Leaves the context
+// locked state untouched; caller holds g_mu.
+int stage_query(const mcr_query* q, Staged* st, bool want_var_windows) {
+  Plan& p = st->plan;
+  p.n = q->n;
+  MCR_TRY(choose_kernel(q, &p));
+  p.r8 = ((q->n + kTileRows - 1) / kTileRows) * kTileRows;
+  p.n_alpha = (q->flags & MCR_F_VAR) ? q->n_alpha : 0;
+  p.per_path = q->S_out || q->loss_out;
+  p.chunk = p.per_path ? (size_t)std::min<uint64_t>((uint64_t)q->paths, kPerPathChunk) : 0;
+  const size_t dev_bytes = layout(p, nullptr, &st->d);
+  MCR_TRY(reserve_arena(dev_bytes));
+  layout(p, (char*)g_ctx.arena, &st->d);
+
+  const size_t n = (size_t)q->n;
+  const size_t in_bytes = align_up((n * n + 2 * n) * sizeof(double));
+  const size_t out_bytes = align_up(sizeof(DevInfo)) + align_up(sizeof(DevAccum)) +
+                           align_up(kHistBins * sizeof(unsigned)) +
+                           align_up((n + 1) * sizeof(unsigned long long));
+  MCR_TRY(reserve_host(in_bytes + out_bytes));
+  st->host = (char*)g_ctx.host;
+  double* hA = (double*)st->host;
+  double* hmu = hA + n * n;
+  double* hw = hmu + n;
+  for (size_t i = 0; i < n; ++i) {
+    std::memcpy(hA + i * n, q->sigma + i * (size_t)q->ld_sigma, n * sizeof(double));
+    hmu[i] = q->mu[i];
+    hw[i] = q->weights ? q->weights[i] : 1.0 / (double)n;
+  }
+  for (size_t i = 0; i < n; ++i)
+    if (!std::isfinite(hmu[i]) || !std::isfinite(hw[i])) {
+      set_error("mu or weights not finite at %zu", i);
+      return MCR_E_ARG;
+    }
+  HIP_TRY(hipMemcpyAsync(st->d.A, hA, n * n * sizeof(double), hipMemcpyHostToDevice, g_ctx.stream));
+  HIP_TRY(hipMemcpyAsync(st->d.mu, hmu, n * sizeof(double), hipMemcpyHostToDevice, g_ctx.stream));
+  HIP_TRY(hipMemcpyAsync(st->d.w, hw, n * sizeof(double), hipMemcpyHostToDevice, g_ctx.stream));
+
+  FactorArgs fa = {};
+  fa.A = st->d.A;
+  fa.mu = st->d.mu;
+  fa.w = st->d.w;
+  fa.L = st->d.L;
+  fa.v64 = st->d.v64;
+  fa.n = q->n;
+  fa.n_alpha = want_var_windows ? p.n_alpha : 0;
+  for (int i = 0; i < fa.n_alpha; ++i) {
+    const double a = q->alpha[i];
+    const double za = inv_norm_cdf(a);
+    const double dens = std::exp(-0.5 * za * za) / std::sqrt(2.0 * M_PI);
+    // six standard errors of the empirical quantile plus a margin for the
+    // fp32 stream's departure from the exact Gaussian
+    fa.za[i] = za;
+    fa.ha[i] = (q->flags & MCR_F_TEST_NARROW) ? 0.0
+                                  : 6.0 * std::sqrt(a * (1.0 - a) / (double)q->paths) / dens + 1e-4;
+  }
+  fa.info = st->d.info;
+  fa.acc = st->d.acc;
+  fa.hist = st->d.hist;
+  HIP_TRY(launch_factor(fa, g_ctx.stream));
+  HIP_TRY(hipMemsetAsync(st->d.s_hist, 0, (n + 1) * sizeof(unsigned long long), g_ctx.stream));
+
+  PrepArgs pa = {};
+  pa.L = st->d.L;
+  pa.mu = st->d.mu;
+  pa.n = q->n;
+  pa.T = p.T;
+  pa.S = p.S;
+  pa.info = st->d.info;
+  pa.v64 = st->d.v64;
+  pa.colpack = st->d.colpack;
+  pa.L32 = st->d.L32;
+  pa.mu32 = st->d.mu32;
+  pa.v32 = st->d.v32;
+  pa.lfrag = st->d.lfrag;
+  pa.rowinit = st->d.rowinit;
+  HIP_TRY(launch_prep(pa, g_ctx.stream));
+  return MCR_OK;
+}
+
+int read_info(const Staged& st, DevInfo* info, DevAccum* acc) {
+  char* out = st.host + align_up((size_t)st.plan.n * st.plan.n * sizeof(double) +
+                                 2 * (size_t)st.plan.n * sizeof(double));
+  DevInfo* hi = (DevInfo*)out;
+  DevAccum* ha = (DevAccum*)(out + align_up(sizeof(DevInfo)));
+  HIP_TRY(hipMemcpyAsync(hi, st.d.info, sizeof(DevInfo), hipMemcpyDeviceToHost, g_ctx.stream));
+  HIP_TRY(hipMemcpyAsync(ha, st.d.acc, sizeof(DevAccum), hipMemcpyDeviceToHost, g_ctx.stream));
+  HIP_TRY(hipStreamSynchronize(g_ctx.stream));
+  *info = *hi;
+  *acc = *ha;
+  return MCR_OK;
+}
+
+int check_factor(const DevInfo& info) {
+  if (info.attempts < 0) {
+    set_error("covariance or mean has non-finite entries");
+    return MCR_E_FACTOR;
+  }
+  if (info.fail_col != 0) {
+    set_error("factorisation failed at column %d even with the clamp fallback", info.fail_col - 1);
+    return MCR_E_FACTOR;
+  }
+  return MCR_OK;
+}
+
+PathArgs base_args(const mcr_query* q, const Staged& st) {
+  PathArgs a = {};
+  a.seed = q->seed;
+  a.n = q->n;
+  a.k = q->k;
+  a.thr = (float)(-q->x);
+  if (q->flags & MCR_F_NEAR) {
+    a.thr_lo = std::nextafter(a.thr, -INFINITY);
+    a.thr_hi = std::nextafter(a.thr, INFINITY);
+  } else {
+    a.thr_lo = 1.0f;
+    a.thr_hi = 0.0f;  // lo > hi: near accounting off
+  }
+  a.colpack = st.d.colpack;
+  a.L32 = st.d.L32;
+  a.mu32 = st.d.mu32;
+  a.v32 = st.d.v32;
+  a.lfrag = st.d.lfrag;
+  a.rowinit = st.d.rowinit;
+  a.T = st.plan.T;
+  a.S = st.plan.S;
+  a.info = st.d.info;
+  a.acc = st.d.acc;
+  a.gather = st.d.gather;
+  a.hist = st.d.hist;
+  return a;
+}
+
+// Runs every path of the query once with the given loss mode.  Per-path
+// outputs (if any) are staged per chunk and copied to the caller's arrays.
+int run_paths(const mcr_query* q, const Staged& st, PathArgs a, bool per_path, bool timed,
+              float* ms) {
+  const uint64_t M = (uint64_t)q->paths;
+  const uint64_t step = per_path ? (uint64_t)st.plan.chunk : kLaunchPaths;
+  if (timed) HIP_TRY(hipEventRecord(g_ctx.ev0, g_ctx.stream));
+  for (uint64_t s = 0; s < M; s += step) {
+    const uint64_t cnt = std::min<uint64_t>(step, M - s);
+    a.path0 = q->path_offset + s;
+    a.npaths = cnt;
+    a.S_out = (per_path && q->S_out) ? st.d.S_out : nullptr;
+    a.loss_out = (per_path && q->loss_out) ? st.d.loss_out : nullptr;
+    HIP_TRY(launch_paths(st.plan.kernel, a, g_ctx.num_cus, g_ctx.stream));
+    if (a.S_out)
+      HIP_TRY(hipMemcpyAsync(q->S_out + s, st.d.S_out, cnt * sizeof(int32_t), hipMemcpyDeviceToHost,
+                             g_ctx.stream));
+    if (a.loss_out)
+      HIP_TRY(hipMemcpyAsync(q->loss_out + s, st.d.loss_out, cnt * sizeof(float),
+                             hipMemcpyDeviceToHost, g_ctx.stream));
+    if (per_path) HIP_TRY(hipStreamSynchronize(g_ctx.stream));
+  }
+  if (timed) {
+    HIP_TRY(hipEventRecord(g_ctx.ev1, g_ctx.stream));
+    HIP_TRY(hipEventSynchronize(g_ctx.ev1));
+    HIP_TRY(hipEventElapsedTime(ms, g_ctx.ev0, g_ctx.ev1));
+  }
+  return MCR_OK;
+}
+
+int64_t fixed32_host(float v) { return (int64_t)((double)v * 4294967296.0); }
+
+// Resolves quantile al from a bracket pass; returns 1 when resolved.
+int try_resolve(const mcr_query* q, const Staged& st, const DevAccum& acc, int slot, int64_t R,
+                double* var, double* es, int64_t* tail) {
+  const int64_t M = q->paths;
+  const int64_t G = (int64_t)acc.gathered[slot];
+  const int64_t above = (int64_t)acc.above[slot];
+  const int64_t below = M - above - G;
+  if (G > (int64_t)kGatherCap || !(below < R && R <= below + G)) return 0;
+  std::vector<float> g((size_t)G);
+  if (G > 0) {
+    if (hipMemcpy(g.data(), st.d.gather + (size_t)slot * kGatherCap, (size_t)G * sizeof(float),
+                  hipMemcpyDeviceToHost) != hipSuccess)
+      return -1;
+  }
+  std::sort(g.begin(), g.end());
+  const int64_t idx = R - below - 1;
+  int64_t t = acc.sum_above[slot];
+  for (int64_t i = idx; i < G; ++i) t += fixed32_host(g[(size_t)i]);
+  *var = (double)g[(size_t)idx];
+  *tail = t;
+  *es = (double)t * std::ldexp(1.0, -32) / (double)(M - R + 1);
+  return 1;
+}
+
+int upload_windows(const Staged& st, const DevInfo& info) {
+  HIP_TRY(hipStreamSynchronize(g_ctx.stream));
+  HIP_TRY(hipMemcpy(st.d.info, &info, sizeof(DevInfo), hipMemcpyHostToDevice));
+  return MCR_OK;
+}
+
+// Quantile refinement when the pass-1 bracket missed the rank (or overflowed
+// the gather buffer): histogram passes narrow the interval, then a bracket
+// pass on the chosen bins.  Each pass regenerates the same paths.
+int refine_quantile(const mcr_query* q, const Staged& st, DevInfo info0, int al, int64_t R,
+                    const DevAccum& acc1, double* var, double* es, int64_t* tail, int* passes) {
+  const int64_t M = q->paths;
+  const double sd = info0.loss_sd > 0 ? info0.loss_sd : 1e-30;
+  const double center = -info0.mp;
+  // interval known to contain rank R
+  double a_lo, a_hi;
+  const int64_t G = (int64_t)acc1.gathered[al], above = (int64_t)acc1.above[al];
+  const int64_t below = M - above - G;
+  if (R <= below) {
+    a_lo = center - 64.0 * sd;
+    a_hi = info0.lo[al];
+  } else if (R > below + G) {
+    a_lo = info0.hi[al];
+    a_hi = center + 64.0 * sd;
+  } else {
+    a_lo = info0.lo[al];
+    a_hi = info0.hi[al];
+  }
+  std::vector<unsigned> hist(kHistBins);
+  for (int iter = 0; iter < 12; ++iter) {
+    if (!(a_hi > a_lo)) a_hi = a_lo + 1e-6 * sd;
+    DevInfo info = info0;
+    info.hist_lo = (float)a_lo;
+    info.hist_scale = (float)(kHistBins / (a_hi - a_lo));
+    MCR_TRY(upload_windows(st, info));
+    HIP_TRY(hipMemsetAsync(st.d.hist, 0, kHistBins * sizeof(unsigned), g_ctx.stream));
+    HIP_TRY(hipMemsetAsync(st.d.acc, 0, sizeof(DevAccum), g_ctx.stream));
+    PathArgs a = base_args(q, st);
+    a.loss_mode = kLossHist;
+    a.thr_lo = 1.0f;
+    a.thr_hi = 0.0f;
+    float ms = 0;
+    MCR_TRY(run_paths(q, st, a, false, false, &ms));
+    ++*passes;
+    DevAccum acc;
+    DevInfo tmp;
+    MCR_TRY(read_info(st, &tmp, &acc));
+    HIP_TRY(hipMemcpy(hist.data(), st.d.hist, kHistBins * sizeof(unsigned), hipMemcpyDeviceToHost));
+    const int64_t hb = (int64_t)acc.hist_below;
+    if (R <= hb) {
+      a_hi = a_lo;
+      a_lo = a_lo - 64.0 * (a_hi - a_lo + sd);
+      continue;
+    }
+    int64_t cum = hb;
+    int bin = -1;
+    for (int b = 0; b < kHistBins; ++b) {
+      if (R <= cum + (int64_t)hist[b]) {
+        bin = b;
+        break;
+      }
+      cum += hist[b];
+    }
+    if (bin < 0) {
+      a_lo = a_hi;
+      a_hi = a_hi + 64.0 * (a_hi - a_lo + sd);
+      continue;
+    }
+    // bracket pass on bins [bin-1, bin+1]
+    const double w = (a_hi - a_lo) / kHistBins;
+    const double lo = a_lo + (bin - 1) * w, hi = a_lo + (bin + 2) * w;
+    int64_t in_bins = 0;
+    for (int b = std::max(0, bin - 1); b <= std::min(kHistBins - 1, bin + 1); ++b) in_bins += hist[b];
+    if (in_bins > (int64_t)kGatherCap) {
+      a_lo = a_lo + bin * w;
+      a_hi = a_lo + w;
+      continue;
+    }
+    info = info0;
+    for (int s = 0; s < kMaxAlpha; ++s) {
+      info.lo[s] = (float)lo;
+      info.hi[s] = (float)hi;
+    }
+    MCR_TRY(upload_windows(st, info));
+    HIP_TRY(hipMemsetAsync(st.d.acc, 0, sizeof(DevAccum), g_ctx.stream));
+    a = base_args(q, st);
+    a.loss_mode = kLossBracket;
+    a.n_alpha = 1;
+    a.thr_lo = 1.0f;
+    a.thr_hi = 0.0f;
+    MCR_TRY(run_paths(q, st, a, false, false, &ms));
+    ++*passes;
+    MCR_TRY(read_info(st, &tmp, &acc));
+    const int r = try_resolve(q, st, acc, 0, R, var, es, tail);
+    if (r < 0) {
+      set_error("copy of gathered losses failed");
+      return MCR_E_DEVICE;
+    }
+    if (r == 1) return MCR_OK;
+    a_lo = lo - w;
+    a_hi = hi + w;
+  }
+  set_error("quantile refinement did not converge for alpha[%d]", al);
+  return MCR_E_DEVICE;
+}
+
+int run_locked(const mcr_query* q, mcr_result* r) {
+  const auto t0 = std::chrono::steady_clock::now();
+  MCR_TRY(validate(q));
+  MCR_TRY(ensure_init(g_ctx.device));
+  HIP_TRY(hipSetDevice(g_ctx.device));
+  std::memset(r, 0, sizeof(*r));
+  Staged st;
+  MCR_TRY(stage_query(q, &st, true));
+
+  PathArgs a = base_args(q, st);
+  a.loss_mode = (q->flags & MCR_F_VAR) ? kLossBracket : kLossNone;
+  a.n_alpha = st.plan.n_alpha;
+  a.s_hist = (q->flags & MCR_F_SHIST) ? st.d.s_hist : nullptr;
+  float ms = 0.0f;
+  MCR_TRY(run_paths(q, st, a, st.plan.per_path, true, &ms));
+  DevInfo info;
+  DevAccum acc;
+  MCR_TRY(read_info(st, &info, &acc));
+  MCR_TRY(check_factor(info));
+
+  const int64_t M = q->paths;
+  r->count = (int64_t)acc.count;
+  r->near_threshold = (int64_t)acc.near;
+  r->p_hat = (double)r->count / (double)M;
+  r->std_error = std::sqrt(r->p_hat * (1.0 - r->p_hat) / (double)M);
+  const double margin = 1.96 * r->std_error;
+  r->ci_lower = std::fmax(0.0, r->p_hat - margin);
+  r->ci_upper = std::fmin(1.0, r->p_hat + margin);
+  r->loss_mean = -info.mp;
+  r->loss_sd = info.loss_sd;
+  r->jitter = info.jitter;
+  r->factor_attempts = info.attempts;
+  r->factor_clamped = info.clamped;
+  r->kernel = st.plan.kernel;
+  r->ms_kernel = ms;
+  r->var_passes = (q->flags & MCR_F_VAR) ? 1 : 0;
+
+  if (q->flags & MCR_F_SHIST) {
+    std::vector<unsigned long long> h((size_t)q->n + 1);
+    HIP_TRY(hipMemcpy(h.data(), st.d.s_hist, h.size() * sizeof(unsigned long long),
+                      hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < h.size(); ++i) q->s_hist[i] = (int64_t)h[i];
+  }
+
+  if (q->flags & MCR_F_VAR) {
+    for (int al = 0; al < st.plan.n_alpha; ++al) {
+      const int64_t R = rank_of(q->alpha[al], M);
+      int res = try_resolve(q, st, acc, al, R, &r->var[al], &r->es[al], &r->tail_sum_fixed[al]);
+      if (res < 0) {
+        set_error("copy of gathered losses failed");
+        return MCR_E_DEVICE;
+      }
+      if (res == 0)
+        MCR_TRY(refine_quantile(q, st, info, al, R, acc, &r->var[al], &r->es[al],
+                                &r->tail_sum_fixed[al], &r->var_passes));
+    }
+  }
+  const auto t1 = std::chrono::steady_clock::now();
+  r->ms_total = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  return MCR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mcr_version(void) { return 1; }
+
+const char* mcr_last_error(void) { return g_err; }
+
+int mcr_init(int device) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (g_ctx.ready && g_ctx.device != device) {
+    set_error("library already initialised on device %d", g_ctx.device);
+    return MCR_E_ARG;
+  }
+  return ensure_init(device);
+}
+
+int mcr_shutdown(void) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!g_ctx.ready) return MCR_OK;
+  HIP_TRY(hipStreamSynchronize(g_ctx.stream));
+  if (g_ctx.arena) HIP_TRY(hipFree(g_ctx.arena));
+  if (g_ctx.host) HIP_TRY(hipHostFree(g_ctx.host));
+  HIP_TRY(hipEventDestroy(g_ctx.ev0));
+  HIP_TRY(hipEventDestroy(g_ctx.ev1));
+  HIP_TRY(hipStreamDestroy(g_ctx.stream));
+  g_ctx = Context();
+  return MCR_OK;
+}
+
+int mcr_run(const mcr_query* q, mcr_result* r) {
+  if (!r) {
+    set_error("result is NULL");
+    return MCR_E_ARG;
+  }
+  std::lock_guard<std::mutex> lk(g_mu);
+  return run_locked(q, r);
+}
+
+int mcr_prepare(const mcr_query* q, mcr_model_export* out) {
+  if (!out) {
+    set_error("export is NULL");
+    return MCR_E_ARG;
+  }
+  std::lock_guard<std::mutex> lk(g_mu);
+  MCR_TRY(validate(q));
+  MCR_TRY(ensure_init(g_ctx.device));
+  HIP_TRY(hipSetDevice(g_ctx.device));
+  mcr_query qq = *q;
+  qq.S_out = nullptr;
+  qq.loss_out = nullptr;
+  Staged st;
+  MCR_TRY(stage_query(&qq, &st, false));
+  DevInfo info;
+  DevAccum acc;
+  MCR_TRY(read_info(st, &info, &acc));
+  MCR_TRY(check_factor(info));
+  const int n = q->n;
+  if (out->L64)
+    HIP_TRY(hipMemcpy(out->L64, st.d.L, (size_t)n * n * sizeof(double), hipMemcpyDeviceToHost));
+  if (out->L32) {
+    std::vector<float> l((size_t)st.plan.r8 * st.plan.r8);
+    HIP_TRY(hipMemcpy(l.data(), st.d.L32, l.size() * sizeof(float), hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) out->L32[(size_t)i * n + j] = l[(size_t)i * st.plan.r8 + j];
+  }
+  if (out->mu32) HIP_TRY(hipMemcpy(out->mu32, st.d.mu32, n * sizeof(float), hipMemcpyDeviceToHost));
+  if (out->v32) HIP_TRY(hipMemcpy(out->v32, st.d.v32, n * sizeof(float), hipMemcpyDeviceToHost));
+  out->mp32 = (float)info.mp;
+  out->thr32 = (float)(-q->x);
+  out->loss_sd = info.loss_sd;
+  return MCR_OK;
+}
+
+int mcr_factor(int n, const double* sigma, int ld_sigma, double* L_out, mcr_factor_info* info) {
+  if (!L_out || !sigma) {
+    set_error("sigma and L_out are required");
+    return MCR_E_ARG;
+  }
+  std::vector<double> mu((size_t)std::max(n, 1), 0.0);
+  mcr_query q = {};
+  q.n = n;
+  q.mu = mu.data();
+  q.sigma = sigma;
+  q.ld_sigma = ld_sigma;
+  q.paths = 1;
+  q.precision = MCR_PREC_FP32;
+  q.kernel = MCR_KERNEL_VALU_TILE;
+  mcr_model_export ex = {};
+  ex.L64 = L_out;
+  std::lock_guard<std::mutex> lk(g_mu);
+  MCR_TRY(validate(&q));
+  MCR_TRY(ensure_init(g_ctx.device));
+  HIP_TRY(hipSetDevice(g_ctx.device));
+  Staged st;
+  MCR_TRY(stage_query(&q, &st, false));
+  DevInfo di;
+  DevAccum acc;
+  MCR_TRY(read_info(st, &di, &acc));
+  MCR_TRY(check_factor(di));
+  HIP_TRY(hipMemcpy(L_out, st.d.L, (size_t)n * n * sizeof(double), hipMemcpyDeviceToHost));
+  if (info) {
+    info->attempts = di.attempts;
+    info->jitter = di.jitter;
+    info->clamped = di.clamped;
+  }
+  return MCR_OK;
+}
+
+int mcr_normals(uint64_t seed, uint64_t path0, int64_t npaths, int n, float* out) {
+  if (!out || npaths < 0 || n < 1) {
+    set_error("bad arguments to mcr_normals");
+    return MCR_E_ARG;
+  }
+  if (npaths == 0) return MCR_OK;
+  std::lock_guard<std::mutex> lk(g_mu);
+  MCR_TRY(ensure_init(g_ctx.device));
+  HIP_TRY(hipSetDevice(g_ctx.device));
+  const size_t bytes = (size_t)npaths * n * sizeof(float);
+  MCR_TRY(reserve_arena(bytes));
+  HIP_TRY(launch_normals(seed, path0, (uint64_t)npaths, n, (float*)g_ctx.arena, g_ctx.stream));
+  HIP_TRY(hipMemcpyAsync(out, g_ctx.arena, bytes, hipMemcpyDeviceToHost, g_ctx.stream));
+  HIP_TRY(hipStreamSynchronize(g_ctx.stream));
+  return MCR_OK;
+}
+
+}  // extern "C"

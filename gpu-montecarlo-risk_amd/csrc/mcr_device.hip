@@ -1,0 +1,839 @@
+// mcr_device.hip -- gfx950 kernels of the Monte Carlo crash-risk model.
+//
+// Replaces monte_carlo_kernel (src/05-GPU/monte_carlos_cuda.cu:23-72) and the host
+// Cholesky it relies on (:84-93).  Per path p the reference computes
+//   Z ~ N(0, I_N), R = mu + L Z (lower-triangular, :51-57), S = #{R_i < -x} (:60-65),
+//   extreme iff S >= k (:69-71), and stores S (:67).
+// Here:
+//   k_factor     FP64 Cholesky-Crout on one workgroup with the jitter policy, then
+//                the portfolio loading v = L^T w, w^T mu and ||v|| (one launch).
+//   k_prep       fp32 / bf16 layouts of the factor for the path kernels.
+//   k_valu_reg   fused path kernel, one path per lane, n <= 64 rows in registers,
+//                factor read through the scalar cache: r_i = mu_i then fmaf(L_ij, z_j)
+//                for j = 0..i, exactly the order the C mirror uses.
+//   k_valu_tile  fused path kernel for n <= 608: 64 paths per workgroup, their
+//                normals staged once in LDS, rows in 8-row register blocks.
+//   k_mfma       32 paths per wave on v_mfma_f32_32x32x16_bf16: A = factor rows
+//                (bf16 hi/lo), B = normals (bf16 hi/lo), three products per step
+//                (hi*hi + hi*lo + lo*hi), fp32 accumulation, n <= 127.
+// Every path kernel reduces the exceedance count, tail counts and fixed-point
+// tail sums per lane, then per wave, with one atomic per wave per counter; no
+// normal ever leaves the chip.
+#include <float.h>
+#include <math.h>
+
+#include "mcr_internal.h"
+#include "mcr_rng.h"
+
+namespace mcr {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+#define MCR_CONSTP(T) const __attribute__((address_space(4))) T*
+
+// Read-only inputs viewed through the constant address space: uniform reads
+// become scalar loads whose results feed VALU ops as SGPR operands.
+template <typename T>
+__device__ __forceinline__ MCR_CONSTP(T) cview(const T* p) {
+  return (MCR_CONSTP(T))p;
+}
+
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+__device__ __forceinline__ long long wave_sum_i64(long long v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+__device__ __forceinline__ double wave_sum_f64(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// Loss -> fixed point in units of 2^-32, truncating (exact for |loss| >= 2^-9).
+__device__ __forceinline__ long long fixed32(float v) {
+  return (long long)((double)v * 4294967296.0);
+}
+
+struct LaneTally {
+  uint32_t count;
+  uint32_t near;
+  uint32_t above[kMaxAlpha];
+  long long sum_above[kMaxAlpha];
+  uint32_t hist_below, hist_above;
+};
+
+// Wave-aggregated append of losses that fall inside window `al`.
+__device__ __forceinline__ void append_loss(const PathArgs& a, int al, bool in, float v) {
+  const unsigned long long m = __ballot(in);
+  if (m == 0ull) return;
+  const int lane = __lane_id();
+  const int leader = __ffsll((long long)m) - 1;
+  const unsigned rank = __popcll(m & ((1ull << lane) - 1ull));
+  unsigned long long base = 0;
+  if (lane == leader) base = atomicAdd(&a.acc->gathered[al], (unsigned long long)__popcll(m));
+  base = __shfl(base, leader);
+  if (in) {
+    const unsigned long long idx = base + rank;
+    if (idx < kGatherCap) a.gather[(size_t)al * kGatherCap + idx] = v;
+  }
+}
+
+// Per-path bookkeeping shared by every path kernel.  Lanes that reach it are
+// active paths; ballots inside see exactly those lanes.
+// Window bounds are re-read from the (scalar-cached) info block each path
+// rather than held in SGPRs across the path loop.
+__device__ __forceinline__ void tally_path(const PathArgs& a, LaneTally& t, uint64_t q, int S,
+                                           float loss, bool near, unsigned* sh_hist) {
+  const DevInfo* ip = a.info;
+  asm volatile("" : "+s"(ip));
+  MCR_CONSTP(DevInfo) w = cview(ip);
+  t.count += (S >= a.k) ? 1u : 0u;
+  t.near += near ? 1u : 0u;
+  if (a.S_out) a.S_out[q] = S;
+  if (a.loss_out) a.loss_out[q] = loss;
+  if (sh_hist) atomicAdd(&sh_hist[S], 1u);
+  if (a.loss_mode == kLossBracket) {
+#pragma unroll
+    for (int al = 0; al < kMaxAlpha; ++al) {
+      if (al < a.n_alpha) {
+        if (loss > w->hi[al]) {
+          t.above[al] += 1u;
+          t.sum_above[al] += fixed32(loss);
+        }
+        append_loss(a, al, loss >= w->lo[al] && loss <= w->hi[al], loss);
+      }
+    }
+  } else if (a.loss_mode == kLossHist) {
+    const float pos = (loss - w->hist_lo) * w->hist_scale;
+    if (pos < 0.0f) {
+      t.hist_below += 1u;
+    } else if (pos >= (float)kHistBins) {
+      t.hist_above += 1u;
+    } else {
+      atomicAdd(&a.hist[(int)pos], 1u);
+    }
+  }
+}
+
+__device__ __forceinline__ void flush_tally(const PathArgs& a, const LaneTally& t) {
+  const int lane = __lane_id();
+  const uint32_t c = wave_sum_u32(t.count);
+  const uint32_t nr = wave_sum_u32(t.near);
+  if (lane == 0) {
+    if (c) atomicAdd(&a.acc->count, (unsigned long long)c);
+    if (nr) atomicAdd(&a.acc->near, (unsigned long long)nr);
+  }
+  if (a.loss_mode == kLossBracket) {
+#pragma unroll
+    for (int al = 0; al < kMaxAlpha; ++al) {
+      if (al < a.n_alpha) {
+        const uint32_t ab = wave_sum_u32(t.above[al]);
+        const long long sm = wave_sum_i64(t.sum_above[al]);
+        if (lane == 0 && ab) {
+          atomicAdd(&a.acc->above[al], (unsigned long long)ab);
+          atomicAdd((unsigned long long*)&a.acc->sum_above[al], (unsigned long long)sm);
+        }
+      }
+    }
+  } else if (a.loss_mode == kLossHist) {
+    const uint32_t b = wave_sum_u32(t.hist_below);
+    const uint32_t ab = wave_sum_u32(t.hist_above);
+    if (lane == 0) {
+      if (b) atomicAdd(&a.acc->hist_below, (unsigned long long)b);
+      if (ab) atomicAdd(&a.acc->hist_above, (unsigned long long)ab);
+    }
+  }
+}
+
+__device__ __forceinline__ void zero_tally(LaneTally& t) {
+  t.count = 0;
+  t.near = 0;
+#pragma unroll
+  for (int i = 0; i < kMaxAlpha; ++i) {
+    t.above[i] = 0;
+    t.sum_above[i] = 0;
+  }
+  t.hist_below = 0;
+  t.hist_above = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Factor: FP64 Cholesky-Crout of A + delta I on one workgroup of 16 waves.
+// Column j: every row i >= j forms A_ij - sum_{k<j} L_ik L_jk (a wave per row,
+// lanes striding k, DPP reduction), then the pivot test, sqrt and scaling.
+// Jitter policy: delta_0 = 0, delta_t = mean(diag) 10^(t-12) for t = 1..8, then
+// the modified factorisation with failing pivots clamped to delta_8.  A pivot
+// fails when it is not above n eps (A_jj + delta).
+// ---------------------------------------------------------------------------
+__device__ double block_sum_f64(double v, double* red) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  v = wave_sum_f64(v);
+  __syncthreads();
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  double s = 0.0;
+  const int nw = (blockDim.x + 63) >> 6;
+  for (int i = 0; i < nw; ++i) s += red[i];
+  return s;
+}
+
+__global__ __launch_bounds__(1024) void k_factor(FactorArgs a) {
+  __shared__ double col[kMaxN];
+  __shared__ double red[16];
+  const int n = a.n;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const double* __restrict__ A = a.A;
+  double* __restrict__ L = a.L;
+
+  // Clear the query's accumulators.
+  if (tid == 0) {
+    DevAccum z = {};
+    *a.acc = z;
+  }
+  for (int i = tid; i < kHistBins; i += blockDim.x) a.hist[i] = 0u;
+
+  double bad = 0.0, dsum = 0.0;
+  for (int e = tid; e < n * n; e += blockDim.x) {
+    const double v = A[e];
+    if (!isfinite(v)) bad = 1.0;
+    L[e] = 0.0;
+  }
+  if (tid < n) dsum = A[tid * n + tid];
+  bad = block_sum_f64(bad, red);
+  dsum = block_sum_f64(dsum, red);
+  if (bad > 0.0) {
+    if (tid == 0) {
+      a.info->attempts = -1;
+      a.info->clamped = 0;
+      a.info->fail_col = 0;
+      a.info->jitter = 0.0;
+    }
+    return;
+  }
+  const double mean = dsum / (double)n;
+
+  int attempts = 0, clamped = 0, fail = 0, last_fail = 0;
+  double delta = 0.0;
+  for (int t = 0; t <= 9; ++t) {
+    const bool clamp = (t == 9);
+    delta = (t == 0) ? 0.0 : mean * pow(10.0, (double)((t < 8 ? t : 8) - 12));
+    fail = 0;
+    clamped = 0;
+    for (int j = 0; j < n && !fail; ++j) {
+      const double* Lj = L + (size_t)j * n;
+      for (int i = j + wave; i < n; i += 16) {
+        const double* Li = L + (size_t)i * n;
+        double p = 0.0;
+        for (int kk = lane; kk < j; kk += 64) p += Li[kk] * Lj[kk];
+        p = wave_sum_f64(p);
+        if (lane == 0) col[i] = A[(size_t)i * n + j] - p;
+      }
+      __syncthreads();
+      const double ajj = A[(size_t)j * n + j] + delta;
+      double d = col[j] + delta;
+      const double tol = (double)n * DBL_EPSILON * ajj;
+      if (!(d > tol) || !isfinite(d)) {
+        if (clamp && isfinite(d)) {
+          d = delta;
+          ++clamped;
+        } else {
+          fail = j + 1;
+        }
+      }
+      if (!fail) {
+        const double ljj = sqrt(d);
+        for (int i = j + tid; i < n; i += blockDim.x)
+          L[(size_t)i * n + j] = (i == j) ? ljj : col[i] / ljj;
+      }
+      __syncthreads();
+    }
+    attempts = t + 1;
+    last_fail = fail;
+    if (!fail) break;
+  }
+
+  // Portfolio loading v = L^T w, mean w^T mu and sd ||v|| in FP64.
+  for (int j = tid; j < n; j += blockDim.x) {
+    double s = 0.0;
+    for (int i = j; i < n; ++i) s += L[(size_t)i * n + j] * a.w[i];
+    a.v64[j] = s;
+  }
+  __syncthreads();
+  double mp = 0.0, ss = 0.0;
+  for (int i = tid; i < n; i += blockDim.x) {
+    mp += a.w[i] * a.mu[i];
+    ss += a.v64[i] * a.v64[i];
+  }
+  mp = block_sum_f64(mp, red);
+  ss = block_sum_f64(ss, red);
+  if (tid == 0) {
+    DevInfo* info = a.info;
+    info->attempts = attempts;
+    info->clamped = clamped;
+    info->fail_col = last_fail;
+    info->jitter = delta;
+    info->mp = mp;
+    const double sd = sqrt(ss);
+    info->loss_sd = sd;
+    for (int al = 0; al < kMaxAlpha; ++al) {
+      if (al < a.n_alpha) {
+        info->lo[al] = (float)(-mp + (a.za[al] - a.ha[al]) * sd);
+        info->hi[al] = (float)(-mp + (a.za[al] + a.ha[al]) * sd);
+      } else {
+        info->lo[al] = 0.0f;
+        info->hi[al] = 0.0f;
+      }
+    }
+    info->hist_lo = 0.0f;
+    info->hist_scale = 0.0f;
+  }
+}
+
+hipError_t launch_factor(const FactorArgs& a, hipStream_t stream) {
+  hipLaunchKernelGGL(k_factor, dim3(1), dim3(1024), 0, stream, a);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Prep: layouts of the factor for the three path kernels.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint16_t bf16_bits(float f) {
+  return __builtin_bit_cast(uint16_t, (__bf16)f);
+}
+__device__ __forceinline__ float bf16_val(uint16_t b) { return as_f32((uint32_t)b << 16); }
+
+__global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
+  const int n = a.n;
+  const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t gsz = (size_t)gridDim.x * blockDim.x;
+  const double* __restrict__ L = a.L;
+  const int r8 = ((n + kTileRows - 1) / kTileRows) * kTileRows;
+  // fp32 loading and mean
+  for (size_t e = gid; e < (size_t)n; e += gsz) {
+    a.v32[e] = (float)a.v64[e];
+    a.mu32[e] = (float)a.mu[e];
+    a.colpack[colpack_offset(n, (int)e) + n - (int)e] = (float)a.v64[e];
+  }
+  // register kernel: column-packed lower triangle
+  for (size_t e = gid; e < (size_t)n * n; e += gsz) {
+    const int i = (int)(e / n), j = (int)(e % n);
+    if (j <= i) a.colpack[colpack_offset(n, j) + i - j] = (float)L[e];
+  }
+  // tile kernel: zero-padded r8 x r8 row-major
+  for (size_t e = gid; e < (size_t)r8 * r8; e += gsz) {
+    const int i = (int)(e / r8), j = (int)(e % r8);
+    a.L32[e] = (i < n && j <= i) ? (float)L[(size_t)i * n + j] : 0.0f;
+  }
+  // MFMA: augmented rows (assets, zero pads, loss row 32T-1 = -v), hi/lo bf16
+  if (a.T > 0) {
+    const int T = a.T, S = a.S;
+    const float mp32 = (float)a.info->mp;
+    for (size_t e = gid; e < (size_t)T * S * 64; e += gsz) {
+      const int lane = (int)(e % 64);
+      const int s = (int)((e / 64) % S);
+      const int t = (int)(e / (64 * (size_t)S));
+      const int row = 32 * t + (lane & 31);
+      const int k0 = 16 * s + 8 * (lane >> 5);
+      uint16_t hb[8], lb[8];
+      for (int q = 0; q < 8; ++q) {
+        const int k = k0 + q;
+        float m = 0.0f;
+        if (k < n) {
+          if (row < n) {
+            m = (k <= row) ? (float)L[(size_t)row * n + k] : 0.0f;
+          } else if (row == 32 * T - 1) {
+            m = -(float)a.v64[k];
+          }
+        }
+        hb[q] = bf16_bits(m);
+        lb[q] = bf16_bits(m - bf16_val(hb[q]));
+      }
+      uint4 h, l;
+      h.x = hb[0] | ((uint32_t)hb[1] << 16);
+      h.y = hb[2] | ((uint32_t)hb[3] << 16);
+      h.z = hb[4] | ((uint32_t)hb[5] << 16);
+      h.w = hb[6] | ((uint32_t)hb[7] << 16);
+      l.x = lb[0] | ((uint32_t)lb[1] << 16);
+      l.y = lb[2] | ((uint32_t)lb[3] << 16);
+      l.z = lb[4] | ((uint32_t)lb[5] << 16);
+      l.w = lb[6] | ((uint32_t)lb[7] << 16);
+      a.lfrag[2 * e] = h;
+      a.lfrag[2 * e + 1] = l;
+    }
+    for (size_t e = gid; e < (size_t)32 * T; e += gsz) {
+      const int row = (int)e;
+      a.rowinit[e] = row < n ? (float)a.mu[row] : (row == 32 * T - 1 ? -mp32 : 0.0f);
+    }
+  }
+}
+
+hipError_t launch_prep(const PrepArgs& a, hipStream_t stream) {
+  size_t work = (size_t)a.n * a.n;
+  const int r8 = ((a.n + kTileRows - 1) / kTileRows) * kTileRows;
+  if ((size_t)r8 * r8 > work) work = (size_t)r8 * r8;
+  int blocks = (int)((work + 255) / 256);
+  if (blocks > 1024) blocks = 1024;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(k_prep, dim3(blocks), dim3(256), 0, stream, a);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Register kernel: one path per lane, N rows in registers (exact N template).
+// Column-oriented: normals arrive four at a time; column j updates rows
+// j..N-1 and the loss row, so row i sees z_0 .. z_i in order.
+// ---------------------------------------------------------------------------
+// Column J of the sweep: rows J..N-1 and the loss row take z_J.
+// The column's address is re-derived through an empty asm so its scalar loads
+// cannot be hoisted above earlier columns: at most ~one column of the factor
+// occupies SGPRs at a time (hoisting them all spills the scalar file).
+template <int N, int J>
+__device__ __forceinline__ void sweep_col(float (&acc)[N], float& g, const float* cp_raw, float zj) {
+  if constexpr (J < N) {
+    constexpr int off = J * (N + 1) - (J * (J - 1)) / 2;
+    asm volatile("" : "+s"(cp_raw));
+    MCR_CONSTP(float) cp = cview(cp_raw);
+#pragma unroll
+    for (int i = J; i < N; ++i) acc[i] = __builtin_fmaf(cp[off + i - J], zj, acc[i]);
+    g = __builtin_fmaf(cp[off + N - J], zj, g);
+  }
+}
+
+// Philox block B supplies z_4B .. z_4B+3; blocks are unrolled at compile time
+// so every accumulator index is a constant and the rows stay in VGPRs.
+template <int N, int B>
+__device__ __forceinline__ void sweep_block(float (&acc)[N], float& g, const float* cp,
+                                            Key2 key, uint64_t path) {
+  if constexpr (4 * B < N) {
+    // per block, so the ten round keys are recomputed by SALU adds rather than
+    // kept (and spilled) across the whole path
+    Key2 bkey = key;
+    asm volatile("" : "+s"(bkey.k0), "+s"(bkey.k1));
+    float z[4];
+    normals4(bkey, path, (uint32_t)B, z);
+    sweep_col<N, 4 * B + 0>(acc, g, cp, z[0]);
+    sweep_col<N, 4 * B + 1>(acc, g, cp, z[1]);
+    sweep_col<N, 4 * B + 2>(acc, g, cp, z[2]);
+    sweep_col<N, 4 * B + 3>(acc, g, cp, z[3]);
+    sweep_block<N, B + 1>(acc, g, cp, key, path);
+  }
+}
+
+template <int N>
+__global__ __launch_bounds__(256) void k_valu_reg(PathArgs a) {
+  __shared__ unsigned sh_hist[kRegMaxN + 1];
+  const bool want_hist = a.s_hist != nullptr;
+  if (want_hist) {
+    for (int i = threadIdx.x; i <= N; i += blockDim.x) sh_hist[i] = 0u;
+    __syncthreads();
+  }
+  const float mp32 = (float)cview(a.info)->mp;
+  const Key2 key = key_of(a.seed);
+  const bool want_near = a.thr_lo < a.thr_hi;
+  LaneTally t;
+  zero_tally(t);
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < a.npaths; q += stride) {
+    const uint64_t path = a.path0 + q;
+    // The factor is loop-invariant; re-deriving its address each path keeps
+    // its ~n^2/2 scalar loads inside the loop instead of hoisted into (and
+    // spilled out of) the scalar register file.
+    // The Philox key schedule is re-derived per path for the same reason.
+    const float* mu_raw = a.mu32;
+    Key2 pkey = key;
+    asm volatile("" : "+s"(mu_raw), "+s"(pkey.k0), "+s"(pkey.k1));
+    MCR_CONSTP(float) mu = cview(mu_raw);
+    float acc[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) acc[i] = mu[i];
+    float g = mp32;
+    sweep_block<N, 0>(acc, g, a.colpack, pkey, path);
+    int S = 0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) S += (acc[i] < a.thr) ? 1 : 0;
+    bool near = false;
+    if (want_near) {
+#pragma unroll
+      for (int i = 0; i < N; ++i) near |= (acc[i] >= a.thr_lo) & (acc[i] <= a.thr_hi);
+    }
+    tally_path(a, t, q, S, -g, near, want_hist ? sh_hist : nullptr);
+  }
+  flush_tally(a, t);
+  if (want_hist) {
+    __syncthreads();
+    for (int i = threadIdx.x; i <= N; i += blockDim.x)
+      if (sh_hist[i]) atomicAdd(&a.s_hist[i], (unsigned long long)sh_hist[i]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Tile kernel: 64 paths per workgroup (lane = path), 4 waves.  Phase 1 writes
+// the paths' normals to LDS as float4 [block][lane]; phase 2 splits the rows
+// into 8-row register blocks dealt to the waves in snake order (block cost grows
+// with the row index), the loss row being one more block.  Row i still sees
+// mu_i, then z_0 .. z_i in order, so results equal the register kernel's.
+// ---------------------------------------------------------------------------
+constexpr int kTileStaticLds = 4 * 64 * 4 + 64 * 4 + (kMaxN + 1) * 4;
+
+__global__ __launch_bounds__(256) void k_valu_tile(PathArgs a) {
+  extern __shared__ float4 zs[];  // [nb][64]
+  __shared__ int sS[4][64];
+  __shared__ float sLoss[64];
+  __shared__ unsigned sh_hist[kMaxN + 1];
+  const int n = a.n;
+  const int nb = (n + 3) >> 2;
+  const int nrb = (n + kTileRows - 1) / kTileRows;
+  const int ldl = nrb * kTileRows;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const bool want_hist = a.s_hist != nullptr;
+  if (want_hist) {
+    for (int i = threadIdx.x; i <= n; i += blockDim.x) sh_hist[i] = 0u;
+  }
+  MCR_CONSTP(float) L = cview(a.L32);
+  MCR_CONSTP(float) mu = cview(a.mu32);
+  MCR_CONSTP(float) v = cview(a.v32);
+  const float mp32 = (float)cview(a.info)->mp;
+  const Key2 key = key_of(a.seed);
+  const bool want_near = a.thr_lo < a.thr_hi;
+  LaneTally t;
+  zero_tally(t);
+
+  for (uint64_t tile = blockIdx.x; tile * kTilePaths < a.npaths; tile += gridDim.x) {
+    const uint64_t q = tile * kTilePaths + lane;
+    const uint64_t path = a.path0 + q;
+    for (int b = wave; b < nb; b += 4) {
+      float z[4];
+      normals4(key, path, (uint32_t)b, z);
+      zs[b * 64 + lane] = make_float4(z[0], z[1], z[2], z[3]);
+    }
+    __syncthreads();
+
+    int S = 0;
+    bool near = false;
+    for (int rb = 0; rb <= nrb; ++rb) {
+      const int p = rb >> 2, r4 = rb & 3;
+      const int owner = (p & 1) ? 3 - r4 : r4;
+      if (owner != wave) continue;
+      if (rb == nrb) {
+        // loss row: g = w^T mu + sum_j v_j z_j, j ascending
+        float g = mp32;
+        for (int jb = 0; jb < nb - 1; ++jb) {
+          const float4 z4 = zs[jb * 64 + lane];
+          g = __builtin_fmaf(v[4 * jb + 0], z4.x, g);
+          g = __builtin_fmaf(v[4 * jb + 1], z4.y, g);
+          g = __builtin_fmaf(v[4 * jb + 2], z4.z, g);
+          g = __builtin_fmaf(v[4 * jb + 3], z4.w, g);
+        }
+        {
+          const int jb = nb - 1;
+          const float4 z4 = zs[jb * 64 + lane];
+          const int rem = n - 4 * jb;
+          g = __builtin_fmaf(v[4 * jb + 0], z4.x, g);
+          if (rem > 1) g = __builtin_fmaf(v[4 * jb + 1], z4.y, g);
+          if (rem > 2) g = __builtin_fmaf(v[4 * jb + 2], z4.z, g);
+          if (rem > 3) g = __builtin_fmaf(v[4 * jb + 3], z4.w, g);
+        }
+        sLoss[lane] = -g;
+        continue;
+      }
+      const int i0 = rb * kTileRows;
+      float acc[kTileRows];
+#pragma unroll
+      for (int r = 0; r < kTileRows; ++r) acc[r] = (i0 + r < n) ? mu[i0 + r] : 0.0f;
+      for (int jb = 0; jb < i0 / 4; ++jb) {
+        const float4 z4 = zs[jb * 64 + lane];
+#pragma unroll
+        for (int r = 0; r < kTileRows; ++r) {
+          MCR_CONSTP(float) Lr = L + (size_t)(i0 + r) * ldl + 4 * jb;
+          acc[r] = __builtin_fmaf(Lr[0], z4.x, acc[r]);
+          acc[r] = __builtin_fmaf(Lr[1], z4.y, acc[r]);
+          acc[r] = __builtin_fmaf(Lr[2], z4.z, acc[r]);
+          acc[r] = __builtin_fmaf(Lr[3], z4.w, acc[r]);
+        }
+      }
+      // diagonal 8x8 block: row i0+r uses columns i0 .. i0+r
+      float zd[8];
+      {
+        const float4 z4 = zs[(i0 / 4) * 64 + lane];
+        zd[0] = z4.x; zd[1] = z4.y; zd[2] = z4.z; zd[3] = z4.w;
+        if (i0 / 4 + 1 < nb) {
+          const float4 z5 = zs[(i0 / 4 + 1) * 64 + lane];
+          zd[4] = z5.x; zd[5] = z5.y; zd[6] = z5.z; zd[7] = z5.w;
+        } else {
+          zd[4] = zd[5] = zd[6] = zd[7] = 0.0f;
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < kTileRows; ++r) {
+        MCR_CONSTP(float) Lr = L + (size_t)(i0 + r) * ldl + i0;
+#pragma unroll
+        for (int c = 0; c <= r; ++c) acc[r] = __builtin_fmaf(Lr[c], zd[c], acc[r]);
+      }
+#pragma unroll
+      for (int r = 0; r < kTileRows; ++r) {
+        if (i0 + r < n) {
+          S += (acc[r] < a.thr) ? 1 : 0;
+          if (want_near) near |= (acc[r] >= a.thr_lo) & (acc[r] <= a.thr_hi);
+        }
+      }
+    }
+    sS[wave][lane] = S | (near ? (1 << 30) : 0);
+    __syncthreads();
+    if (wave == 0 && q < a.npaths) {
+      int tot = 0;
+      bool nr = false;
+#pragma unroll
+      for (int w = 0; w < 4; ++w) {
+        const int e = sS[w][lane];
+        tot += e & 0xffff;
+        nr |= (e >> 30) & 1;
+      }
+      tally_path(a, t, q, tot, sLoss[lane], nr, want_hist ? sh_hist : nullptr);
+    }
+    __syncthreads();
+  }
+  flush_tally(a, t);
+  if (want_hist) {
+    __syncthreads();
+    for (int i = threadIdx.x; i <= n; i += blockDim.x)
+      if (sh_hist[i]) atomicAdd(&a.s_hist[i], (unsigned long long)sh_hist[i]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// MFMA kernel: T tiles of 32 rows (assets, zero pads, loss row 32T-1), S steps
+// of 16 normals.  Lane l of a wave owns path l%32 of the wave's 32-path group
+// and normals 16s + 8(l/32) .. +7 of that path, which is exactly the B operand
+// layout of v_mfma_f32_32x32x16_bf16; the factor fragments (A operand) stay in
+// registers for the whole kernel.  D[row][path]: lane l holds path l%32, rows
+// 8(r/4) + 4(l/32) + r%4.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t pack_bf16x2(float lo, float hi) {
+  return (uint32_t)bf16_bits(lo) | ((uint32_t)bf16_bits(hi) << 16);
+}
+
+__device__ __forceinline__ void split8(const float z[8], bf16x8& hi, bf16x8& lo) {
+  uint32_t h[4], l[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    h[q] = pack_bf16x2(z[2 * q], z[2 * q + 1]);
+    const float r0 = z[2 * q] - as_f32(h[q] << 16);
+    const float r1 = z[2 * q + 1] - as_f32(h[q] & 0xffff0000u);
+    l[q] = pack_bf16x2(r0, r1);
+  }
+  uint4 hv = make_uint4(h[0], h[1], h[2], h[3]);
+  uint4 lv = make_uint4(l[0], l[1], l[2], l[3]);
+  hi = __builtin_bit_cast(bf16x8, hv);
+  lo = __builtin_bit_cast(bf16x8, lv);
+}
+
+template <int T>
+__global__ __launch_bounds__(256) void k_mfma(PathArgs a) {
+  constexpr int SMAX = 2 * T;
+  __shared__ unsigned sh_hist[32 * kMfmaMaxT];
+  const int lane = threadIdx.x & 63;
+  const int g = lane >> 5, col = lane & 31;
+  const int n = a.n;
+  const int S_all = a.S;
+  const int nb = (n + 3) >> 2;
+  const bool want_hist = a.s_hist != nullptr;
+  if (want_hist) {
+    for (int i = threadIdx.x; i <= n; i += blockDim.x) sh_hist[i] = 0u;
+    __syncthreads();
+  }
+  const Key2 key = key_of(a.seed);
+
+  bf16x8 Lhi[T][SMAX], Llo[T][SMAX];
+#pragma unroll
+  for (int t = 0; t < T; ++t) {
+#pragma unroll
+    for (int s = 0; s < SMAX; ++s) {
+      const int steps = (t == T - 1) ? S_all : (2 * t + 2 < S_all ? 2 * t + 2 : S_all);
+      if (s < steps) {
+        const size_t e = ((size_t)(t * S_all + s) * 64 + lane) * 2;
+        Lhi[t][s] = __builtin_bit_cast(bf16x8, a.lfrag[e]);
+        Llo[t][s] = __builtin_bit_cast(bf16x8, a.lfrag[e + 1]);
+      } else {
+        Lhi[t][s] = __builtin_bit_cast(bf16x8, make_uint4(0, 0, 0, 0));
+        Llo[t][s] = Lhi[t][s];
+      }
+    }
+  }
+  f32x16 init[T];
+#pragma unroll
+  for (int t = 0; t < T; ++t)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) init[t][r] = a.rowinit[32 * t + 8 * (r >> 2) + 4 * g + (r & 3)];
+  float thr_last[16];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int row = 32 * (T - 1) + 8 * (r >> 2) + 4 * g + (r & 3);
+    thr_last[r] = row < n ? a.thr : -INFINITY;
+  }
+
+  LaneTally t;
+  zero_tally(t);
+  const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x >> 6);
+  for (uint64_t grp = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+       grp * 32 < a.npaths; grp += waves) {
+    const uint64_t q = grp * 32 + col;
+    const uint64_t path = a.path0 + q;
+    bf16x8 zhi[SMAX], zlo[SMAX];
+#pragma unroll
+    for (int s = 0; s < SMAX; ++s) {
+      if (s < S_all) {
+        float z[8];
+        const int b0 = 4 * s + 2 * g;
+        if (b0 < nb) {
+          normals4(key, path, (uint32_t)b0, z);
+        } else {
+          z[0] = z[1] = z[2] = z[3] = 0.0f;
+        }
+        if (b0 + 1 < nb) {
+          normals4(key, path, (uint32_t)(b0 + 1), z + 4);
+        } else {
+          z[4] = z[5] = z[6] = z[7] = 0.0f;
+        }
+        split8(z, zhi[s], zlo[s]);
+      }
+    }
+    f32x16 acc[T];
+#pragma unroll
+    for (int tt = 0; tt < T; ++tt) {
+      const int steps = (tt == T - 1) ? S_all : (2 * tt + 2 < S_all ? 2 * tt + 2 : S_all);
+      acc[tt] = init[tt];
+#pragma unroll
+      for (int s = 0; s < SMAX; ++s) {
+        if (s < steps) {
+          acc[tt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Llo[tt][s], zhi[s], acc[tt], 0, 0, 0);
+          acc[tt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Lhi[tt][s], zlo[s], acc[tt], 0, 0, 0);
+          acc[tt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Lhi[tt][s], zhi[s], acc[tt], 0, 0, 0);
+        }
+      }
+    }
+    int Sl = 0;
+#pragma unroll
+    for (int tt = 0; tt < T - 1; ++tt)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) Sl += (acc[tt][r] < a.thr) ? 1 : 0;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) Sl += (acc[T - 1][r] < thr_last[r]) ? 1 : 0;
+    const int S = Sl + __shfl_xor(Sl, 32);
+    const float loss = __shfl_xor(acc[T - 1][15], 32);
+    if (g == 0 && q < a.npaths) tally_path(a, t, q, S, loss, false, want_hist ? sh_hist : nullptr);
+  }
+  flush_tally(a, t);
+  if (want_hist) {
+    __syncthreads();
+    for (int i = threadIdx.x; i <= n; i += blockDim.x)
+      if (sh_hist[i]) atomicAdd(&a.s_hist[i], (unsigned long long)sh_hist[i]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Debug: the normal stream itself, one path per lane.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_normals(uint64_t seed, uint64_t path0, uint64_t npaths,
+                                                 int n, float* out) {
+  const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= npaths) return;
+  const Key2 key = key_of(seed);
+  for (int b = 0; 4 * b < n; ++b) {
+    float z[4];
+    normals4(key, path0 + q, (uint32_t)b, z);
+    for (int u = 0; u < 4; ++u)
+      if (4 * b + u < n) out[q * n + 4 * b + u] = z[u];
+  }
+}
+
+hipError_t launch_normals(uint64_t seed, uint64_t path0, uint64_t npaths, int n, float* out,
+                          hipStream_t stream) {
+  const uint64_t blocks = (npaths + 255) / 256;
+  hipLaunchKernelGGL(k_normals, dim3((unsigned)blocks), dim3(256), 0, stream, seed, path0, npaths,
+                     n, out);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Dispatch.
+// ---------------------------------------------------------------------------
+typedef void (*PathKernel)(PathArgs);
+
+template <int N>
+struct RegTable {
+  static void fill(PathKernel* t) {
+    t[N] = k_valu_reg<N>;
+    RegTable<N - 1>::fill(t);
+  }
+};
+template <>
+struct RegTable<0> {
+  static void fill(PathKernel* t) { t[0] = nullptr; }
+};
+
+static PathKernel reg_kernel(int n) {
+  static PathKernel table[kRegMaxN + 1];
+  static bool init = false;
+  if (!init) {
+    RegTable<kRegMaxN>::fill(table);
+    init = true;
+  }
+  return (n >= 1 && n <= kRegMaxN) ? table[n] : nullptr;
+}
+
+static PathKernel mfma_kernel(int T) {
+  switch (T) {
+    case 1: return k_mfma<1>;
+    case 2: return k_mfma<2>;
+    case 3: return k_mfma<3>;
+    case 4: return k_mfma<4>;
+    default: return nullptr;
+  }
+}
+
+static uint64_t min_u64(uint64_t x, uint64_t y) { return x < y ? x : y; }
+
+hipError_t launch_paths(int kernel, const PathArgs& a, int num_cus, hipStream_t stream) {
+  if (a.npaths == 0) return hipSuccess;
+  if (kernel == 1) {
+    PathKernel k = reg_kernel(a.n);
+    if (!k) return hipErrorInvalidValue;
+    const uint64_t blocks = min_u64((a.npaths + 255) / 256, (uint64_t)num_cus * 8);
+    hipLaunchKernelGGL(k, dim3((unsigned)blocks), dim3(256), 0, stream, a);
+    return hipGetLastError();
+  }
+  if (kernel == 2) {
+    if (a.n < 1 || a.n > kMaxN) return hipErrorInvalidValue;
+    const int nb = (a.n + 3) >> 2;
+    const size_t lds = (size_t)nb * 64 * sizeof(float4);
+    static bool attr = false;
+    if (!attr) {
+      hipError_t e = hipFuncSetAttribute((const void*)k_valu_tile,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         160 * 1024 - kTileStaticLds);
+      if (e != hipSuccess) return e;
+      attr = true;
+    }
+    uint64_t per_cu = (160 * 1024) / (lds + kTileStaticLds);
+    if (per_cu < 1) per_cu = 1;
+    if (per_cu > 8) per_cu = 8;
+    const uint64_t blocks = min_u64((a.npaths + kTilePaths - 1) / kTilePaths, (uint64_t)num_cus * per_cu);
+    hipLaunchKernelGGL(k_valu_tile, dim3((unsigned)blocks), dim3(256), lds, stream, a);
+    return hipGetLastError();
+  }
+  if (kernel == 3) {
+    PathKernel k = mfma_kernel(a.T);
+    if (!k) return hipErrorInvalidValue;
+    const uint64_t blocks = min_u64((a.npaths + 127) / 128, (uint64_t)num_cus * 8);
+    hipLaunchKernelGGL(k, dim3((unsigned)blocks), dim3(256), 0, stream, a);
+    return hipGetLastError();
+  }
+  return hipErrorInvalidValue;
+}
+
+}  // namespace mcr

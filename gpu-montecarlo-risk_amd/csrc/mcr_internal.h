@@ -1,0 +1,113 @@
+// mcr_internal.h -- structures shared by the kernels (mcr_device.hip) and the
+// C-ABI implementation (mcr_api.cpp).  Not part of the public ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mcr {
+
+constexpr int kMaxAlpha = 4;
+constexpr int kMaxN = 608;      // tile kernel: 64 paths x n normals in LDS (152 KiB)
+constexpr int kRegMaxN = 64;    // register kernel: one row accumulator per asset
+constexpr int kMfmaMaxT = 4;    // MFMA kernel: 32-row tiles (assets + loss row)
+constexpr int kTilePaths = 64;  // paths per workgroup in the tile kernel
+constexpr int kTileRows = 8;    // rows per register block in the tile kernel
+constexpr unsigned kGatherCap = 1u << 18;  // loss values kept per VaR window
+constexpr int kHistBins = 4096;            // refinement histogram
+
+enum LossMode : int { kLossNone = 0, kLossBracket = 1, kLossHist = 2 };
+
+// Written on the device by the factor kernel, read by the path kernels and
+// copied back to the host with the accumulators.
+struct DevInfo {
+  int attempts;      // 1..9, 10 = clamp fallback, -1 = non-finite input
+  int clamped;       // pivots clamped
+  int fail_col;      // 0, or first failing column + 1 of the final attempt
+  int pad0;
+  double jitter;     // delta added to the diagonal
+  double mp;         // w^T mu (FP64)
+  double loss_sd;    // ||L^T w|| (FP64)
+  float lo[kMaxAlpha];  // loss windows [lo, hi] per VaR level
+  float hi[kMaxAlpha];
+  float hist_lo;        // refinement histogram origin and inverse bin width
+  float hist_scale;
+};
+
+// Device accumulators, cleared by the factor kernel before each query.
+struct DevAccum {
+  unsigned long long count;                 // paths with S >= k
+  unsigned long long near;                  // paths with a row within 1 ulp of thr
+  unsigned long long above[kMaxAlpha];      // losses > hi
+  long long sum_above[kMaxAlpha];           // fixed-point (2^-32) sum of those
+  unsigned long long gathered[kMaxAlpha];   // losses in [lo, hi] (may exceed cap)
+  unsigned long long hist_below;            // refinement: losses < hist_lo
+  unsigned long long hist_above;            // refinement: losses past the last bin
+};
+
+// Kernel arguments (passed by value).
+struct PathArgs {
+  uint64_t seed;
+  uint64_t path0;   // id of path 0 of this launch
+  uint64_t npaths;  // paths in this launch
+  int n, k;
+  float thr;        // (float)(-x)
+  float thr_lo, thr_hi;  // thr -/+ 1 ulp (near-threshold accounting)
+  const float* colpack;  // register kernel: column j = L[j..n-1][j], v[j]
+  const float* L32;      // tile kernel: row-major n x n fp32 factor
+  const float* mu32;     // [n]
+  const float* v32;      // [n] L^T w
+  const uint4* lfrag;    // MFMA: [T][S][64 lanes][hi, lo] 8 x bf16 each
+  const float* rowinit;  // MFMA: [32 T] row initial values
+  int T, S;              // MFMA tiles and 16-wide K steps
+  const DevInfo* info;
+  DevAccum* acc;
+  int32_t* S_out;        // optional [npaths]
+  float* loss_out;       // optional [npaths]
+  unsigned long long* s_hist;  // optional [n+1]
+  int loss_mode;
+  int n_alpha;
+  float* gather;         // [n_alpha][kGatherCap]
+  unsigned* hist;        // [kHistBins] refinement
+};
+
+// Factor kernel arguments.
+struct FactorArgs {
+  const double* A;   // [n*n] covariance (row-major, ld = n)
+  const double* mu;  // [n]
+  const double* w;   // [n]
+  double* L;         // [n*n] out
+  double* v64;       // [n] out: L^T w
+  int n;
+  int n_alpha;
+  double za[kMaxAlpha];  // inverse normal cdf at alpha
+  double ha[kMaxAlpha];  // window half-widths in units of loss_sd
+  DevInfo* info;
+  DevAccum* acc;
+  unsigned* hist;        // cleared too
+};
+
+struct PrepArgs {
+  const double* L;   // [n*n]
+  const double* mu;  // [n]
+  int n, T, S;
+  const DevInfo* info;
+  double* v64;       // [n] scratch: L^T w
+  float* colpack;
+  float* L32;
+  float* mu32;
+  float* v32;
+  uint4* lfrag;
+  float* rowinit;
+};
+
+// Launchers (mcr_device.hip).  All asynchronous on `stream`.
+hipError_t launch_factor(const FactorArgs& a, hipStream_t stream);
+hipError_t launch_prep(const PrepArgs& a, hipStream_t stream);
+hipError_t launch_paths(int kernel, const PathArgs& a, int num_cus, hipStream_t stream);
+hipError_t launch_normals(uint64_t seed, uint64_t path0, uint64_t npaths, int n, float* out,
+                          hipStream_t stream);
+
+__host__ __device__ inline int colpack_offset(int n, int j) { return j * (n + 1) - (j * (j - 1)) / 2; }
+__host__ __device__ inline size_t colpack_size(int n) { return (size_t)colpack_offset(n, n); }
+
+}  // namespace mcr

@@ -1,0 +1,151 @@
+/*
+ * mcr_hip.h -- C ABI of the MI355X (gfx950) Monte Carlo crash-risk library.
+ *
+ * Plain C types and host pointers only.  Every entry point returns MCR_OK (0)
+ * or a negative MCR_E_* code; mcr_last_error() describes the last failure of
+ * the calling thread.  Nothing here aborts or exits.
+ *
+ * Reference interfaces each entry point replaces:
+ *   get_hip_model / hip_simulate  -> get_cuda_model / cuda_simulate
+ *       (src/05-GPU/monte_carlos_cuda.cu:165-171, :75-163), dispatched by
+ *       src/main_runner.c:198-241 through ModelFunctions (src/model_interface.h:33-36)
+ *   mcr_run                      -> the body of cuda_simulate: Cholesky (:84-93),
+ *       flatten + H2D (:96-122), monte_carlo_kernel launch (:125-129, kernel :23-72),
+ *       D2H (:138-143) and statistics (:146-150), with the count reduced on device
+ *   mcr_factor                   -> gsl_linalg_cholesky_decomp1 as called at
+ *       src/05-GPU/monte_carlos_cuda.cu:87-93 (FP64, here on the GPU with jitter)
+ *   mcr_normals                  -> curand_init + curand_normal_double
+ *       (src/05-GPU/monte_carlos_cuda.cu:30-31, :45-47): the per-path normal stream
+ *   mcr_synth_factor_model       -> the mu/Sigma producer preprocessing/preprocessing_stats.py:5-28
+ *       (sample mean, ddof=1 covariance), fed by a synthetic factor model
+ */
+#ifndef MCR_HIP_H
+#define MCR_HIP_H
+
+#include <stdint.h>
+
+#include "model_interface.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MCR_OK 0
+#define MCR_E_ARG -1         /* invalid argument */
+#define MCR_E_UNSUPPORTED -2 /* size or option outside what a kernel supports */
+#define MCR_E_DEVICE -3      /* HIP runtime error or no device */
+#define MCR_E_NOMEM -4       /* device or host allocation failed */
+#define MCR_E_FACTOR -5      /* covariance not factorable (non-finite input) */
+
+#define MCR_MAX_ALPHA 4
+#define MCR_MAX_N 608 /* largest asset count any kernel accepts */
+
+/* precision / kernel family */
+#define MCR_PREC_FP32 0   /* fused fp32 VALU kernels: bit-exact with the C mirror */
+#define MCR_PREC_BF16X3 1 /* MFMA bf16 hi/lo split products, fp32 accumulation */
+#define MCR_PREC_AUTO 2   /* fastest supported for the size */
+
+/* explicit kernel override (0 = choose from precision) */
+#define MCR_KERNEL_AUTO 0
+#define MCR_KERNEL_VALU_REG 1  /* one path per lane, rows in registers, n <= 64 */
+#define MCR_KERNEL_VALU_TILE 2 /* 64 paths per workgroup, normals in LDS, n <= 608 */
+#define MCR_KERNEL_MFMA 3      /* 32 paths per wave on 32x32x16 bf16 MFMA, n <= 127 */
+
+/* flags */
+#define MCR_F_VAR 0x1   /* portfolio-loss quantiles (VaR) and tail sums at alpha[] */
+#define MCR_F_NEAR 0x2  /* count paths with some r_i within 1 ulp of the threshold */
+#define MCR_F_SHIST 0x4 /* histogram of S (n+1 bins) into s_hist */
+/* test hook: zero-width VaR windows, forcing the quantile refinement passes */
+#define MCR_F_TEST_NARROW 0x100
+
+typedef struct mcr_query {
+  int n;                /* assets, 1 <= n <= MCR_MAX_N */
+  const double *mu;     /* [n] mean daily returns */
+  const double *sigma;  /* [n * ld_sigma] covariance, row-major */
+  int ld_sigma;         /* row stride of sigma (>= n) */
+  const double *weights; /* [n] portfolio weights, NULL = equal weights 1/n */
+  double x;             /* crash threshold: asset i crashes iff r_i < -x */
+  int k;                /* extreme event iff #crashes >= k */
+  int64_t paths;        /* Monte Carlo paths M (>= 1) */
+  uint64_t seed;        /* Philox key */
+  uint64_t path_offset; /* id of the first path (paths are keyed by id) */
+  int precision;        /* MCR_PREC_* */
+  int kernel;           /* MCR_KERNEL_* override, 0 = auto */
+  int flags;            /* MCR_F_* */
+  int n_alpha;          /* number of VaR levels (MCR_F_VAR), <= MCR_MAX_ALPHA */
+  double alpha[MCR_MAX_ALPHA];
+  int32_t *S_out;       /* optional host [paths]: crash count per path */
+  float *loss_out;      /* optional host [paths]: portfolio loss per path */
+  int64_t *s_hist;      /* optional host [n+1] (MCR_F_SHIST) */
+} mcr_query;
+
+typedef struct mcr_result {
+  int64_t count;          /* paths with S >= k */
+  int64_t near_threshold; /* MCR_F_NEAR: paths with some |r_i - thr| <= 1 ulp */
+  double p_hat, std_error, ci_lower, ci_upper;
+  /* MCR_F_VAR: VaR_a = ceil(a M)-th smallest loss; es = mean of the
+   * M - ceil(a M) + 1 largest losses; tail_sum in units of 2^-32 */
+  double var[MCR_MAX_ALPHA];
+  double es[MCR_MAX_ALPHA];
+  int64_t tail_sum_fixed[MCR_MAX_ALPHA];
+  int var_passes;         /* path passes used for the quantiles (1 normally) */
+  double loss_mean;       /* w^T mu negated: mean of the simulated loss */
+  double loss_sd;         /* ||L^T w||: sd of the simulated loss */
+  double jitter;          /* delta added to the diagonal before factoring */
+  int factor_attempts;    /* 1 = plain Cholesky succeeded */
+  int factor_clamped;     /* pivots clamped by the last-resort fallback */
+  int kernel;             /* MCR_KERNEL_* that ran */
+  double ms_kernel;       /* path kernel time (HIP events) */
+  double ms_total;        /* host wall time of the call */
+} mcr_result;
+
+typedef struct mcr_factor_info {
+  int attempts;  /* 1..9 jitter attempts; 10 = clamp fallback ran */
+  double jitter; /* delta finally added to the diagonal */
+  int clamped;   /* pivots replaced by delta in the fallback */
+} mcr_factor_info;
+
+/* fp32 model the path kernels read, for parity checks (host copies) */
+typedef struct mcr_model_export {
+  double *L64;  /* [n*n] FP64 factor, row-major, upper zero (or NULL) */
+  float *L32;   /* [n*n] fp32 factor (or NULL) */
+  float *mu32;  /* [n] (or NULL) */
+  float *v32;   /* [n] L^T w in fp32 (or NULL) */
+  float mp32;   /* w^T mu in fp32 */
+  float thr32;  /* (float)(-x) */
+  double loss_sd;
+} mcr_model_export;
+
+/* Library lifetime.  mcr_init is optional (first use initialises device 0). */
+int mcr_init(int device);
+int mcr_shutdown(void);
+const char *mcr_last_error(void);
+int mcr_version(void);
+
+/* One risk query: factor, prepare, run all paths, reduce, statistics. */
+int mcr_run(const mcr_query *q, mcr_result *r);
+
+/* GPU FP64 Cholesky of sigma with the jitter policy; L_out [n*n] row-major. */
+int mcr_factor(int n, const double *sigma, int ld_sigma, double *L_out, mcr_factor_info *info);
+
+/* Factor + prepare only; copies the device model to the caller's arrays. */
+int mcr_prepare(const mcr_query *q, mcr_model_export *out);
+
+/* The device normal stream: out[p*n + j] = z_j of path path0 + p. */
+int mcr_normals(uint64_t seed, uint64_t path0, int64_t npaths, int n, float *out);
+
+/* Synthetic factor-model data (host): r_t = drift + B f_t + e_t, t < T, with
+ * B ~ U(0.2, 0.8) [n x K], f_t ~ N(0, fvol^2 I_K), e_t ~ N(0, ivol^2 I_n);
+ * mu = sample mean, sigma = ddof-1 sample covariance [n*n].  returns (optional)
+ * receives the [T*n] return matrix. */
+int mcr_synth_factor_model(int n, int K, int T, double drift, double fvol, double ivol,
+                           uint64_t seed, double *mu, double *sigma, double *returns);
+
+/* Plug-in model for the reference driver (src/main_runner.c). */
+int hip_simulate(MonteCarloParams *params, MonteCarloResult *result);
+ModelFunctions get_hip_model(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MCR_HIP_H */

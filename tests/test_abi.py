@@ -1,0 +1,65 @@
+"""C-ABI surface (CPU): the built library loads, exports every function that
+include/mcr_hip.h declares, and the host-only data generator behaves.  No GPU
+call is made here."""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+
+def test_library_exports_every_declared_function(mcr):
+    names = mcr.declared_functions()
+    assert {"mcr_run", "mcr_factor", "get_hip_model", "hip_simulate"} <= set(names)
+    L = ctypes.CDLL(mcr.LIB_PATH)
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_model_interface_header_layout(mcr):
+    # field offsets of the GSL-layout structs the reference driver passes in
+    assert ctypes.sizeof(mcr.GslVector) == 40
+    assert mcr.GslMatrix.data.offset == 24
+    assert mcr.MonteCarloParams.M.offset == 32
+    assert mcr.MonteCarloResult.S_values.offset == 16
+
+
+def test_synth_matches_numpy_moments(mcr):
+    mu, sigma, R = mcr.synth_factor_model(12, 3, 700, 77, want_returns=True)
+    assert np.allclose(mu, R.mean(axis=0), rtol=0, atol=1e-15)
+    assert np.allclose(sigma, np.cov(R, rowvar=False, ddof=1), rtol=1e-12, atol=1e-18)
+    assert np.array_equal(sigma, sigma.T)
+    assert np.all(np.linalg.eigvalsh(sigma) > 0)
+
+
+def test_synth_is_deterministic_and_seeded(mcr):
+    a = mcr.synth_factor_model(8, 2, 300, 5)
+    b = mcr.synth_factor_model(8, 2, 300, 5)
+    c = mcr.synth_factor_model(8, 2, 300, 6)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    assert not np.array_equal(a[1], c[1])
+
+
+def test_synth_factor_structure(mcr):
+    # population covariance is B B^T fvol^2 + ivol^2 I with B ~ U(0.2, 0.8):
+    # mean diagonal ~ K E[B^2] 1e-4 + 1e-4
+    n, K, T = 40, 5, 20000
+    mu, sigma = mcr.synth_factor_model(n, K, T, 1002)
+    eb2 = (0.8**3 - 0.2**3) / (3 * 0.6)
+    expect_diag = K * eb2 * 1e-4 + 1e-4
+    assert np.mean(np.diag(sigma)) == pytest.approx(expect_diag, rel=0.15)
+    assert np.mean(mu) == pytest.approx(2e-4, abs=3 * np.sqrt(np.mean(np.diag(sigma)) / T))
+
+
+@pytest.mark.skipif(os.environ.get("MCR_SLOW") != "1", reason="n=600 generator (set MCR_SLOW=1)")
+def test_synth_large_condition_number(mcr):
+    mu, sigma = mcr.synth_factor_model(600, 20, 2500, 1003)
+    c = np.linalg.cond(sigma)
+    assert 1e3 < c < 1e6
+
+
+def test_bad_arguments_rejected_without_gpu(mcr):
+    with pytest.raises(mcr.McrError):
+        mcr.synth_factor_model(0, 2, 100, 1)

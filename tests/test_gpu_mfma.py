@@ -1,0 +1,55 @@
+"""bf16x3 MFMA kernel against the FP64 oracle on the identical stream, and
+statistical agreement on independent streams.
+
+Tolerances (stated here, checked below):
+  * exceedance count: |count_mfma - count_fp64| <= 3 + 2e-4 * count_fp64
+    (flips only for paths with some asset within ~1e-6 of the threshold);
+  * VaR at 0.99 / 0.999: relative error vs the FP64 order statistic <= 1e-4;
+  * independent seeds: |P_a - P_b| <= 3 sqrt(SE_a^2 + SE_b^2)."""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from helpers import equicorrelated_sigma, equicorrelated_tail, factor_like, rank_of
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("n", [1, 5, 20, 31, 32, 50, 63, 64, 100, 127])
+def test_mfma_vs_fp64_same_stream(mcr, orc, n):
+    M, seed, x = 200_000, 77, 0.02
+    k = max(1, n // 5)
+    mu, sigma = factor_like(n, min(5, n), 40 + n)
+    r = mcr.run(mu, sigma, x, k, M, seed, precision=mcr.PREC_BF16X3, alphas=(0.99, 0.999),
+                want_S=True)
+    assert r.kernel == "mfma"
+    L, *_ = mcr.factor(sigma)
+    w = np.full(n, 1.0 / n)
+    ref = orc.simulate_f64(L, mu, w, x, seed, 0, M, k, want_S=True, want_loss=True, nthreads=8)
+    assert abs(r.count - ref["count"]) <= 3 + 2e-4 * ref["count"], (r.count, ref["count"])
+    assert np.mean(r.S != ref["S"]) <= 2e-3
+    srt = np.sort(ref["loss"])
+    for i, a in enumerate((0.99, 0.999)):
+        v64 = srt[rank_of(a, M) - 1]
+        assert abs(r.var[i] - v64) <= 1e-4 * abs(v64), (a, r.var[i], v64)
+
+
+def test_mfma_independent_streams_agree_with_fp32(mcr):
+    mu, sigma = factor_like(50, 5, 1002)
+    a = mcr.run(mu, sigma, 0.02, 10, 20_000_000, 1, precision=mcr.PREC_BF16X3)
+    b = mcr.run(mu, sigma, 0.02, 10, 20_000_000, 2, precision=mcr.PREC_FP32)
+    assert abs(a.p_hat - b.p_hat) <= 3 * math.sqrt(a.std_error**2 + b.std_error**2)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_closed_form_on_gpu(mcr, precision):
+    """Reference prototype's example (python_serial.py:112-122) at 1e7 paths."""
+    n, k, x, var, rho = 10, 5, 0.2, 0.04, 0.3
+    exact = equicorrelated_tail(n, k, x, 0.0, var, rho)
+    r = mcr.run(np.zeros(n), equicorrelated_sigma(n, var, rho), x, k, 10_000_000, 67,
+                precision=precision)
+    se = math.sqrt(exact * (1 - exact) / 10_000_000)
+    assert abs(r.p_hat - exact) < 4 * se, (r.p_hat, exact)
