@@ -110,9 +110,9 @@ __device__ __forceinline__ void tally_path(const PathArgs& a, LaneTally& t, uint
     }
   } else if (a.loss_mode == kLossHist) {
     const float pos = (loss - w->hist_lo) * w->hist_scale;
-    if (pos < 0.0f) {
+    if (!(pos >= 0.0f)) {  // also catches NaN
       t.hist_below += 1u;
-    } else if (pos >= (float)kHistBins) {
+    } else if (!(pos < (float)kHistBins)) {
       t.hist_above += 1u;
     } else {
       atomicAdd(&a.hist[(int)pos], 1u);
@@ -813,9 +813,10 @@ hipError_t launch_paths(int kernel, const PathArgs& a, int num_cus, hipStream_t 
     const size_t lds = (size_t)nb * 64 * sizeof(float4);
     static bool attr = false;
     if (!attr) {
+      // the largest dynamic request (n = kMaxN); static LDS comes on top
+      const int max_dyn = ((kMaxN + 3) / 4) * 64 * (int)sizeof(float4);
       hipError_t e = hipFuncSetAttribute((const void*)k_valu_tile,
-                                         hipFuncAttributeMaxDynamicSharedMemorySize,
-                                         160 * 1024 - kTileStaticLds);
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, max_dyn);
       if (e != hipSuccess) return e;
       attr = true;
     }

@@ -66,7 +66,8 @@ def test_factor_jitter_policy_matches_oracle(mcr, orc):
     L, att, jit, cl = mcr.factor(A)
     oatt, oL, ojit, ocl = orc.factor(A)
     assert (att, cl) == (oatt, ocl) and att > 1
-    assert jit == ojit
+    # same policy step; the diagonal mean is summed in another order
+    assert jit == pytest.approx(ojit, rel=1e-13)
     assert np.allclose(L, oL, rtol=1e-8, atol=1e-12)
 
 
@@ -74,7 +75,8 @@ def test_factor_clamp_fallback(mcr, orc):
     A = np.diag([1.0, 1.0, -0.5, 1.0]) * 1e-4
     L, att, jit, cl = mcr.factor(A)
     oatt, oL, ojit, ocl = orc.factor(A)
-    assert att == oatt == 10 and cl == ocl == 1 and jit == ojit
+    assert att == oatt == 10 and cl == ocl == 1
+    assert jit == pytest.approx(ojit, rel=1e-13)
     assert np.allclose(L, oL)
 
 
