@@ -33,6 +33,8 @@ def parse():
     p.add_argument("--precision", default="auto", choices=["auto", "fp32", "bf16x3"])
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--cpu-seconds", type=float, default=12.0)
+    p.add_argument("--copy-mb", type=int, default=0,
+                   help="first run a plain device copy of this many MiB (counter calibration)")
     return p.parse_args()
 
 
@@ -72,6 +74,15 @@ def main():
     import __graft_entry__ as ge
 
     mcr = ge.load_package()
+    if args.copy_mb:
+        import torch
+
+        src = torch.ones(args.copy_mb << 18, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"# calibration copy: {args.copy_mb} MiB read + {args.copy_mb} MiB written",
+              file=sys.stderr, flush=True)
     W = WORKLOAD
     n = W["n"]
     mu, sigma = mcr.synth_factor_model(n, W["factors"], W["days"], W["data_seed"])

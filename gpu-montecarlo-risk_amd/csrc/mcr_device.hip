@@ -19,148 +19,9 @@
 // Every path kernel reduces the exceedance count, tail counts and fixed-point
 // tail sums per lane, then per wave, with one atomic per wave per counter; no
 // normal ever leaves the chip.
-#include <float.h>
-#include <math.h>
-
-#include "mcr_internal.h"
-#include "mcr_rng.h"
+#include "mcr_kern_common.h"
 
 namespace mcr {
-
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float f32x16 __attribute__((ext_vector_type(16)));
-#define MCR_CONSTP(T) const __attribute__((address_space(4))) T*
-
-// Read-only inputs viewed through the constant address space: uniform reads
-// become scalar loads whose results feed VALU ops as SGPR operands.
-template <typename T>
-__device__ __forceinline__ MCR_CONSTP(T) cview(const T* p) {
-  return (MCR_CONSTP(T))p;
-}
-
-__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-  return v;
-}
-__device__ __forceinline__ long long wave_sum_i64(long long v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-  return v;
-}
-__device__ __forceinline__ double wave_sum_f64(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-  return v;
-}
-
-// Loss -> fixed point in units of 2^-32, truncating (exact for |loss| >= 2^-9).
-__device__ __forceinline__ long long fixed32(float v) {
-  return (long long)((double)v * 4294967296.0);
-}
-
-struct LaneTally {
-  uint32_t count;
-  uint32_t near;
-  uint32_t above[kMaxAlpha];
-  long long sum_above[kMaxAlpha];
-  uint32_t hist_below, hist_above;
-};
-
-// Wave-aggregated append of losses that fall inside window `al`.
-__device__ __forceinline__ void append_loss(const PathArgs& a, int al, bool in, float v) {
-  const unsigned long long m = __ballot(in);
-  if (m == 0ull) return;
-  const int lane = __lane_id();
-  const int leader = __ffsll((long long)m) - 1;
-  const unsigned rank = __popcll(m & ((1ull << lane) - 1ull));
-  unsigned long long base = 0;
-  if (lane == leader) base = atomicAdd(&a.acc->gathered[al], (unsigned long long)__popcll(m));
-  base = __shfl(base, leader);
-  if (in) {
-    const unsigned long long idx = base + rank;
-    if (idx < kGatherCap) a.gather[(size_t)al * kGatherCap + idx] = v;
-  }
-}
-
-// Per-path bookkeeping shared by every path kernel.  Lanes that reach it are
-// active paths; ballots inside see exactly those lanes.
-// Window bounds are re-read from the (scalar-cached) info block each path
-// rather than held in SGPRs across the path loop.
-__device__ __forceinline__ void tally_path(const PathArgs& a, LaneTally& t, uint64_t q, int S,
-                                           float loss, bool near, unsigned* sh_hist) {
-  const DevInfo* ip = a.info;
-  asm volatile("" : "+s"(ip));
-  MCR_CONSTP(DevInfo) w = cview(ip);
-  t.count += (S >= a.k) ? 1u : 0u;
-  t.near += near ? 1u : 0u;
-  if (a.S_out) a.S_out[q] = S;
-  if (a.loss_out) a.loss_out[q] = loss;
-  if (sh_hist) atomicAdd(&sh_hist[S], 1u);
-  if (a.loss_mode == kLossBracket) {
-#pragma unroll
-    for (int al = 0; al < kMaxAlpha; ++al) {
-      if (al < a.n_alpha) {
-        if (loss > w->hi[al]) {
-          t.above[al] += 1u;
-          t.sum_above[al] += fixed32(loss);
-        }
-        append_loss(a, al, loss >= w->lo[al] && loss <= w->hi[al], loss);
-      }
-    }
-  } else if (a.loss_mode == kLossHist) {
-    const float pos = (loss - w->hist_lo) * w->hist_scale;
-    if (!(pos >= 0.0f)) {  // also catches NaN
-      t.hist_below += 1u;
-    } else if (!(pos < (float)kHistBins)) {
-      t.hist_above += 1u;
-    } else {
-      atomicAdd(&a.hist[(int)pos], 1u);
-    }
-  }
-}
-
-__device__ __forceinline__ void flush_tally(const PathArgs& a, const LaneTally& t) {
-  const int lane = __lane_id();
-  const uint32_t c = wave_sum_u32(t.count);
-  const uint32_t nr = wave_sum_u32(t.near);
-  if (lane == 0) {
-    if (c) atomicAdd(&a.acc->count, (unsigned long long)c);
-    if (nr) atomicAdd(&a.acc->near, (unsigned long long)nr);
-  }
-  if (a.loss_mode == kLossBracket) {
-#pragma unroll
-    for (int al = 0; al < kMaxAlpha; ++al) {
-      if (al < a.n_alpha) {
-        const uint32_t ab = wave_sum_u32(t.above[al]);
-        const long long sm = wave_sum_i64(t.sum_above[al]);
-        if (lane == 0 && ab) {
-          atomicAdd(&a.acc->above[al], (unsigned long long)ab);
-          atomicAdd((unsigned long long*)&a.acc->sum_above[al], (unsigned long long)sm);
-        }
-      }
-    }
-  } else if (a.loss_mode == kLossHist) {
-    const uint32_t b = wave_sum_u32(t.hist_below);
-    const uint32_t ab = wave_sum_u32(t.hist_above);
-    if (lane == 0) {
-      if (b) atomicAdd(&a.acc->hist_below, (unsigned long long)b);
-      if (ab) atomicAdd(&a.acc->hist_above, (unsigned long long)ab);
-    }
-  }
-}
-
-__device__ __forceinline__ void zero_tally(LaneTally& t) {
-  t.count = 0;
-  t.near = 0;
-#pragma unroll
-  for (int i = 0; i < kMaxAlpha; ++i) {
-    t.above[i] = 0;
-    t.sum_above[i] = 0;
-  }
-  t.hist_below = 0;
-  t.hist_above = 0;
-}
 
 // ---------------------------------------------------------------------------
 // Factor: FP64 Cholesky-Crout of A + delta I on one workgroup of 16 waves.
@@ -302,11 +163,6 @@ hipError_t launch_factor(const FactorArgs& a, hipStream_t stream) {
 // ---------------------------------------------------------------------------
 // Prep: layouts of the factor for the three path kernels.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint16_t bf16_bits(float f) {
-  return __builtin_bit_cast(uint16_t, (__bf16)f);
-}
-__device__ __forceinline__ float bf16_val(uint16_t b) { return as_f32((uint32_t)b << 16); }
-
 __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
   const int n = a.n;
   const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -367,7 +223,7 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
     }
     for (size_t e = gid; e < (size_t)32 * T; e += gsz) {
       const int row = (int)e;
-      a.rowinit[e] = row < n ? (float)a.mu[row] : (row == 32 * T - 1 ? -mp32 : 0.0f);
+      a.rowinit[e] = row < n ? (float)a.mu[row] : (row == 32 * T - 1 ? -mp32 : INFINITY);
     }
   }
 }
@@ -381,94 +237,6 @@ hipError_t launch_prep(const PrepArgs& a, hipStream_t stream) {
   if (blocks < 1) blocks = 1;
   hipLaunchKernelGGL(k_prep, dim3(blocks), dim3(256), 0, stream, a);
   return hipGetLastError();
-}
-
-// ---------------------------------------------------------------------------
-// Register kernel: one path per lane, N rows in registers (exact N template).
-// Column-oriented: normals arrive four at a time; column j updates rows
-// j..N-1 and the loss row, so row i sees z_0 .. z_i in order.
-// ---------------------------------------------------------------------------
-// Column J of the sweep: rows J..N-1 and the loss row take z_J.
-// The column's address is re-derived through an empty asm so its scalar loads
-// cannot be hoisted above earlier columns: at most ~one column of the factor
-// occupies SGPRs at a time (hoisting them all spills the scalar file).
-template <int N, int J>
-__device__ __forceinline__ void sweep_col(float (&acc)[N], float& g, const float* cp_raw, float zj) {
-  if constexpr (J < N) {
-    constexpr int off = J * (N + 1) - (J * (J - 1)) / 2;
-    asm volatile("" : "+s"(cp_raw));
-    MCR_CONSTP(float) cp = cview(cp_raw);
-#pragma unroll
-    for (int i = J; i < N; ++i) acc[i] = __builtin_fmaf(cp[off + i - J], zj, acc[i]);
-    g = __builtin_fmaf(cp[off + N - J], zj, g);
-  }
-}
-
-// Philox block B supplies z_4B .. z_4B+3; blocks are unrolled at compile time
-// so every accumulator index is a constant and the rows stay in VGPRs.
-template <int N, int B>
-__device__ __forceinline__ void sweep_block(float (&acc)[N], float& g, const float* cp,
-                                            Key2 key, uint64_t path) {
-  if constexpr (4 * B < N) {
-    // per block, so the ten round keys are recomputed by SALU adds rather than
-    // kept (and spilled) across the whole path
-    Key2 bkey = key;
-    asm volatile("" : "+s"(bkey.k0), "+s"(bkey.k1));
-    float z[4];
-    normals4(bkey, path, (uint32_t)B, z);
-    sweep_col<N, 4 * B + 0>(acc, g, cp, z[0]);
-    sweep_col<N, 4 * B + 1>(acc, g, cp, z[1]);
-    sweep_col<N, 4 * B + 2>(acc, g, cp, z[2]);
-    sweep_col<N, 4 * B + 3>(acc, g, cp, z[3]);
-    sweep_block<N, B + 1>(acc, g, cp, key, path);
-  }
-}
-
-template <int N>
-__global__ __launch_bounds__(256) void k_valu_reg(PathArgs a) {
-  __shared__ unsigned sh_hist[kRegMaxN + 1];
-  const bool want_hist = a.s_hist != nullptr;
-  if (want_hist) {
-    for (int i = threadIdx.x; i <= N; i += blockDim.x) sh_hist[i] = 0u;
-    __syncthreads();
-  }
-  const float mp32 = (float)cview(a.info)->mp;
-  const Key2 key = key_of(a.seed);
-  const bool want_near = a.thr_lo < a.thr_hi;
-  LaneTally t;
-  zero_tally(t);
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < a.npaths; q += stride) {
-    const uint64_t path = a.path0 + q;
-    // The factor is loop-invariant; re-deriving its address each path keeps
-    // its ~n^2/2 scalar loads inside the loop instead of hoisted into (and
-    // spilled out of) the scalar register file.
-    // The Philox key schedule is re-derived per path for the same reason.
-    const float* mu_raw = a.mu32;
-    Key2 pkey = key;
-    asm volatile("" : "+s"(mu_raw), "+s"(pkey.k0), "+s"(pkey.k1));
-    MCR_CONSTP(float) mu = cview(mu_raw);
-    float acc[N];
-#pragma unroll
-    for (int i = 0; i < N; ++i) acc[i] = mu[i];
-    float g = mp32;
-    sweep_block<N, 0>(acc, g, a.colpack, pkey, path);
-    int S = 0;
-#pragma unroll
-    for (int i = 0; i < N; ++i) S += (acc[i] < a.thr) ? 1 : 0;
-    bool near = false;
-    if (want_near) {
-#pragma unroll
-      for (int i = 0; i < N; ++i) near |= (acc[i] >= a.thr_lo) & (acc[i] <= a.thr_hi);
-    }
-    tally_path(a, t, q, S, -g, near, want_hist ? sh_hist : nullptr);
-  }
-  flush_tally(a, t);
-  if (want_hist) {
-    __syncthreads();
-    for (int i = threadIdx.x; i <= N; i += blockDim.x)
-      if (sh_hist[i]) atomicAdd(&a.s_hist[i], (unsigned long long)sh_hist[i]);
-  }
 }
 
 // ---------------------------------------------------------------------------
@@ -606,136 +374,6 @@ __global__ __launch_bounds__(256) void k_valu_tile(PathArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// MFMA kernel: T tiles of 32 rows (assets, zero pads, loss row 32T-1), S steps
-// of 16 normals.  Lane l of a wave owns path l%32 of the wave's 32-path group
-// and normals 16s + 8(l/32) .. +7 of that path, which is exactly the B operand
-// layout of v_mfma_f32_32x32x16_bf16; the factor fragments (A operand) stay in
-// registers for the whole kernel.  D[row][path]: lane l holds path l%32, rows
-// 8(r/4) + 4(l/32) + r%4.
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t pack_bf16x2(float lo, float hi) {
-  return (uint32_t)bf16_bits(lo) | ((uint32_t)bf16_bits(hi) << 16);
-}
-
-__device__ __forceinline__ void split8(const float z[8], bf16x8& hi, bf16x8& lo) {
-  uint32_t h[4], l[4];
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    h[q] = pack_bf16x2(z[2 * q], z[2 * q + 1]);
-    const float r0 = z[2 * q] - as_f32(h[q] << 16);
-    const float r1 = z[2 * q + 1] - as_f32(h[q] & 0xffff0000u);
-    l[q] = pack_bf16x2(r0, r1);
-  }
-  uint4 hv = make_uint4(h[0], h[1], h[2], h[3]);
-  uint4 lv = make_uint4(l[0], l[1], l[2], l[3]);
-  hi = __builtin_bit_cast(bf16x8, hv);
-  lo = __builtin_bit_cast(bf16x8, lv);
-}
-
-template <int T>
-__global__ __launch_bounds__(256) void k_mfma(PathArgs a) {
-  constexpr int SMAX = 2 * T;
-  __shared__ unsigned sh_hist[32 * kMfmaMaxT];
-  const int lane = threadIdx.x & 63;
-  const int g = lane >> 5, col = lane & 31;
-  const int n = a.n;
-  const int S_all = a.S;
-  const int nb = (n + 3) >> 2;
-  const bool want_hist = a.s_hist != nullptr;
-  if (want_hist) {
-    for (int i = threadIdx.x; i <= n; i += blockDim.x) sh_hist[i] = 0u;
-    __syncthreads();
-  }
-  const Key2 key = key_of(a.seed);
-
-  bf16x8 Lhi[T][SMAX], Llo[T][SMAX];
-#pragma unroll
-  for (int t = 0; t < T; ++t) {
-#pragma unroll
-    for (int s = 0; s < SMAX; ++s) {
-      const int steps = (t == T - 1) ? S_all : (2 * t + 2 < S_all ? 2 * t + 2 : S_all);
-      if (s < steps) {
-        const size_t e = ((size_t)(t * S_all + s) * 64 + lane) * 2;
-        Lhi[t][s] = __builtin_bit_cast(bf16x8, a.lfrag[e]);
-        Llo[t][s] = __builtin_bit_cast(bf16x8, a.lfrag[e + 1]);
-      } else {
-        Lhi[t][s] = __builtin_bit_cast(bf16x8, make_uint4(0, 0, 0, 0));
-        Llo[t][s] = Lhi[t][s];
-      }
-    }
-  }
-  f32x16 init[T];
-#pragma unroll
-  for (int t = 0; t < T; ++t)
-#pragma unroll
-    for (int r = 0; r < 16; ++r) init[t][r] = a.rowinit[32 * t + 8 * (r >> 2) + 4 * g + (r & 3)];
-  float thr_last[16];
-#pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    const int row = 32 * (T - 1) + 8 * (r >> 2) + 4 * g + (r & 3);
-    thr_last[r] = row < n ? a.thr : -INFINITY;
-  }
-
-  LaneTally t;
-  zero_tally(t);
-  const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x >> 6);
-  for (uint64_t grp = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-       grp * 32 < a.npaths; grp += waves) {
-    const uint64_t q = grp * 32 + col;
-    const uint64_t path = a.path0 + q;
-    bf16x8 zhi[SMAX], zlo[SMAX];
-#pragma unroll
-    for (int s = 0; s < SMAX; ++s) {
-      if (s < S_all) {
-        float z[8];
-        const int b0 = 4 * s + 2 * g;
-        if (b0 < nb) {
-          normals4(key, path, (uint32_t)b0, z);
-        } else {
-          z[0] = z[1] = z[2] = z[3] = 0.0f;
-        }
-        if (b0 + 1 < nb) {
-          normals4(key, path, (uint32_t)(b0 + 1), z + 4);
-        } else {
-          z[4] = z[5] = z[6] = z[7] = 0.0f;
-        }
-        split8(z, zhi[s], zlo[s]);
-      }
-    }
-    f32x16 acc[T];
-#pragma unroll
-    for (int tt = 0; tt < T; ++tt) {
-      const int steps = (tt == T - 1) ? S_all : (2 * tt + 2 < S_all ? 2 * tt + 2 : S_all);
-      acc[tt] = init[tt];
-#pragma unroll
-      for (int s = 0; s < SMAX; ++s) {
-        if (s < steps) {
-          acc[tt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Llo[tt][s], zhi[s], acc[tt], 0, 0, 0);
-          acc[tt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Lhi[tt][s], zlo[s], acc[tt], 0, 0, 0);
-          acc[tt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Lhi[tt][s], zhi[s], acc[tt], 0, 0, 0);
-        }
-      }
-    }
-    int Sl = 0;
-#pragma unroll
-    for (int tt = 0; tt < T - 1; ++tt)
-#pragma unroll
-      for (int r = 0; r < 16; ++r) Sl += (acc[tt][r] < a.thr) ? 1 : 0;
-#pragma unroll
-    for (int r = 0; r < 16; ++r) Sl += (acc[T - 1][r] < thr_last[r]) ? 1 : 0;
-    const int S = Sl + __shfl_xor(Sl, 32);
-    const float loss = __shfl_xor(acc[T - 1][15], 32);
-    if (g == 0 && q < a.npaths) tally_path(a, t, q, S, loss, false, want_hist ? sh_hist : nullptr);
-  }
-  flush_tally(a, t);
-  if (want_hist) {
-    __syncthreads();
-    for (int i = threadIdx.x; i <= n; i += blockDim.x)
-      if (sh_hist[i]) atomicAdd(&a.s_hist[i], (unsigned long long)sh_hist[i]);
-  }
-}
-
-// ---------------------------------------------------------------------------
 // Debug: the normal stream itself, one path per lane.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_normals(uint64_t seed, uint64_t path0, uint64_t npaths,
@@ -762,40 +400,6 @@ hipError_t launch_normals(uint64_t seed, uint64_t path0, uint64_t npaths, int n,
 // ---------------------------------------------------------------------------
 // Dispatch.
 // ---------------------------------------------------------------------------
-typedef void (*PathKernel)(PathArgs);
-
-template <int N>
-struct RegTable {
-  static void fill(PathKernel* t) {
-    t[N] = k_valu_reg<N>;
-    RegTable<N - 1>::fill(t);
-  }
-};
-template <>
-struct RegTable<0> {
-  static void fill(PathKernel* t) { t[0] = nullptr; }
-};
-
-static PathKernel reg_kernel(int n) {
-  static PathKernel table[kRegMaxN + 1];
-  static bool init = false;
-  if (!init) {
-    RegTable<kRegMaxN>::fill(table);
-    init = true;
-  }
-  return (n >= 1 && n <= kRegMaxN) ? table[n] : nullptr;
-}
-
-static PathKernel mfma_kernel(int T) {
-  switch (T) {
-    case 1: return k_mfma<1>;
-    case 2: return k_mfma<2>;
-    case 3: return k_mfma<3>;
-    case 4: return k_mfma<4>;
-    default: return nullptr;
-  }
-}
-
 static uint64_t min_u64(uint64_t x, uint64_t y) { return x < y ? x : y; }
 
 hipError_t launch_paths(int kernel, const PathArgs& a, int num_cus, hipStream_t stream) {
@@ -828,9 +432,9 @@ hipError_t launch_paths(int kernel, const PathArgs& a, int num_cus, hipStream_t 
     return hipGetLastError();
   }
   if (kernel == 3) {
-    PathKernel k = mfma_kernel(a.T);
+    PathKernel k = mfma_kernel(a.n, a.T);
     if (!k) return hipErrorInvalidValue;
-    const uint64_t blocks = min_u64((a.npaths + 127) / 128, (uint64_t)num_cus * 8);
+    const uint64_t blocks = min_u64((a.npaths + 255) / 256, (uint64_t)num_cus * 8);
     hipLaunchKernelGGL(k, dim3((unsigned)blocks), dim3(256), 0, stream, a);
     return hipGetLastError();
   }

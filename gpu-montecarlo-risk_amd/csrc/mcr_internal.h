@@ -10,6 +10,7 @@ constexpr int kMaxAlpha = 4;
 constexpr int kMaxN = 608;      // tile kernel: 64 paths x n normals in LDS (152 KiB)
 constexpr int kRegMaxN = 64;    // register kernel: one row accumulator per asset
 constexpr int kMfmaMaxT = 4;    // MFMA kernel: 32-row tiles (assets + loss row)
+constexpr int kMfmaNB = 32;     // MFMA kernel: Philox blocks per path (n <= 127)
 constexpr int kTilePaths = 64;  // paths per workgroup in the tile kernel
 constexpr int kTileRows = 8;    // rows per register block in the tile kernel
 constexpr unsigned kGatherCap = 1u << 18;  // loss values kept per VaR window

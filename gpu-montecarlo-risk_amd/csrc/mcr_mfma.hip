@@ -1,0 +1,209 @@
+// mcr_mfma.hip -- the bf16x3 MFMA path kernel (n <= 127).
+#include "mcr_kern_common.h"
+
+namespace mcr {
+
+// ---------------------------------------------------------------------------
+// MFMA kernel: T tiles of 32 rows (assets, zero pads, loss row 32T-1), S steps
+// of 16 normals, v_mfma_f32_32x32x16_bf16 with A = factor rows (bf16 hi/lo,
+// resident in registers for the whole kernel) and B = normals (bf16 hi/lo).
+// A wave handles 64 paths, lane l generating every normal of path l.  For
+// step s, X = z[16s..16s+7] and Y = z[16s+8..16s+15] of the lane's own path;
+// one v_permlane32_swap(X, Y) turns them into the B operands of the two
+// 32-path tiles ([X_lo | Y_lo] for paths 0-31, [X_hi | Y_hi] for paths
+// 32-63).  D[row][path]: lane l holds path l%32 of a tile and rows
+// 8(r/4) + 4(l/32) + r%4; two more swaps bring each path's crash count and
+// loss back to lane l.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t pack_bf16x2(float lo, float hi) {
+  return (uint32_t)bf16_bits(lo) | ((uint32_t)bf16_bits(hi) << 16);
+}
+
+// z[0..7] -> bf16 hi and lo parts, two per 32-bit word
+__device__ __forceinline__ void split8(const float* z, uint32_t hi[4], uint32_t lo[4]) {
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    hi[q] = pack_bf16x2(z[2 * q], z[2 * q + 1]);
+    const float r0 = z[2 * q] - as_f32(hi[q] << 16);
+    const float r1 = z[2 * q + 1] - as_f32(hi[q] & 0xffff0000u);
+    lo[q] = pack_bf16x2(r0, r1);
+  }
+}
+
+// lanes 32-63 of x <-> lanes 0-31 of y
+__device__ __forceinline__ void swap32(uint32_t& x, uint32_t& y) {
+  const auto r = __builtin_amdgcn_permlane32_swap(x, y, false, false);
+  x = r[0];
+  y = r[1];
+}
+
+__device__ __forceinline__ bf16x8 as_bf16x8(const uint32_t w[4]) {
+  return __builtin_bit_cast(bf16x8, make_uint4(w[0], w[1], w[2], w[3]));
+}
+
+// NB = Philox blocks per path (n <= 4 NB), T = row tiles; both compile-time
+// so every block, step and tile bound below is a constant.
+template <int NB, int T>
+__global__ __launch_bounds__(256) void k_mfma(PathArgs a) {
+  constexpr int S = (NB + 3) / 4;  // 16-wide K steps
+  __shared__ unsigned sh_hist[32 * kMfmaMaxT];
+  const int lane = threadIdx.x & 63;
+  const int g = lane >> 5;
+  const int n = a.n;
+  const bool want_hist = a.s_hist != nullptr;
+  if (want_hist) {
+    for (int i = threadIdx.x; i <= n; i += blockDim.x) sh_hist[i] = 0u;
+    __syncthreads();
+  }
+  const Key2 key = key_of(a.seed);
+  // Factor fragments are streamed per step from L1/L2 (16 B per lane per
+  // tile and part) instead of being held for the whole kernel.
+  const uint4* __restrict__ lf = a.lfrag + lane * 2;
+  // Row initial values (pads +inf so they never count; the loss row, last
+  // tile r = 15 in the upper half-wave, is excluded explicitly).
+  const float4* __restrict__ rinit = (const float4*)a.rowinit;
+
+  LaneTally t;
+  zero_tally(t);
+  const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x >> 6);
+  for (uint64_t grp = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+       grp * 64 < a.npaths; grp += waves) {
+    const uint64_t q = grp * 64 + lane;
+    const uint64_t path = a.path0 + q;
+    f32x16 accA[T], accB[T];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      bf16x8 Lhi[T], Llo[T];
+#pragma unroll
+      for (int tt = 0; tt < T; ++tt) {
+        const int steps = (tt == T - 1) ? S : (2 * tt + 2 < S ? 2 * tt + 2 : S);
+        if (s < steps) {
+          const size_t e = (size_t)(tt * S + s) * 128;
+          Lhi[tt] = __builtin_bit_cast(bf16x8, lf[e]);
+          Llo[tt] = __builtin_bit_cast(bf16x8, lf[e + 1]);
+        }
+      }
+      float z[16];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (4 * s + u < NB) {
+          normals4(key, path, (uint32_t)(4 * s + u), &z[4 * u]);
+        } else {
+          z[4 * u] = z[4 * u + 1] = z[4 * u + 2] = z[4 * u + 3] = 0.0f;
+        }
+      }
+      uint32_t xh[4], xl[4], yh[4], yl[4];
+      split8(z, xh, xl);
+      split8(z + 8, yh, yl);
+#pragma unroll
+      for (int q4 = 0; q4 < 4; ++q4) {
+        swap32(xh[q4], yh[q4]);
+        swap32(xl[q4], yl[q4]);
+      }
+      const bf16x8 ah = as_bf16x8(xh), al = as_bf16x8(xl);  // paths 0-31
+      const bf16x8 bh = as_bf16x8(yh), bl = as_bf16x8(yl);  // paths 32-63
+#pragma unroll
+      for (int tt = 0; tt < T; ++tt) {
+        const int steps = (tt == T - 1) ? S : (2 * tt + 2 < S ? 2 * tt + 2 : S);
+        if (s < steps) {
+          if (s == 0) {
+            f32x16 init;
+#pragma unroll
+            for (int r4 = 0; r4 < 4; ++r4) {
+              const float4 v = rinit[(32 * tt + 8 * r4 + 4 * g) >> 2];
+              init[4 * r4 + 0] = v.x;
+              init[4 * r4 + 1] = v.y;
+              init[4 * r4 + 2] = v.z;
+              init[4 * r4 + 3] = v.w;
+            }
+            accA[tt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Llo[tt], ah, init, 0, 0, 0);
+            accB[tt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Llo[tt], bh, init, 0, 0, 0);
+          } else {
+            accA[tt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Llo[tt], ah, accA[tt], 0, 0, 0);
+            accB[tt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Llo[tt], bh, accB[tt], 0, 0, 0);
+          }
+          accA[tt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Lhi[tt], al, accA[tt], 0, 0, 0);
+          accB[tt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Lhi[tt], bl, accB[tt], 0, 0, 0);
+          accA[tt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Lhi[tt], ah, accA[tt], 0, 0, 0);
+          accB[tt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Lhi[tt], bh, accB[tt], 0, 0, 0);
+        }
+      }
+    }
+    uint32_t SA = 0, SB = 0;
+#pragma unroll
+    for (int tt = 0; tt < T; ++tt)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        if (tt == T - 1 && r == 15) {
+          SA += (accA[tt][r] < a.thr && g == 0) ? 1u : 0u;
+          SB += (accB[tt][r] < a.thr && g == 0) ? 1u : 0u;
+        } else {
+          SA += (accA[tt][r] < a.thr) ? 1u : 0u;
+          SB += (accB[tt][r] < a.thr) ? 1u : 0u;
+        }
+      }
+    swap32(SA, SB);
+    const int Sp = (int)(SA + SB);
+    uint32_t la = as_u32(accA[T - 1][15]), lb = as_u32(accB[T - 1][15]);
+    swap32(la, lb);
+    const float loss = as_f32(lb);
+    if (q < a.npaths) tally_path(a, t, q, Sp, loss, false, want_hist ? sh_hist : nullptr);
+  }
+  flush_tally(a, t);
+  if (want_hist) {
+    __syncthreads();
+    for (int i = threadIdx.x; i <= n; i += blockDim.x)
+      if (sh_hist[i]) atomicAdd(&a.s_hist[i], (unsigned long long)sh_hist[i]);
+  }
+}
+
+// MFMA instantiations: NB = ceil(n/4) Philox blocks, T = ceil((n+1)/32) tiles.
+// This file is compiled once per part (MCR_MFMA_PART = 0..3), each part
+// instantiating eight consecutive NB, so the instantiations build in parallel.
+#ifndef MCR_MFMA_PART
+#define MCR_MFMA_PART 0
+#endif
+constexpr int kNbLo = 8 * MCR_MFMA_PART + 1;
+constexpr int kNbHi = 8 * MCR_MFMA_PART + 8;
+
+template <int NB>
+struct MfmaFill {
+  static void fill(PathKernel (*t)[kMfmaMaxT + 1]) {
+    constexpr int Tlo = (4 * NB - 2 + 31) / 32;  // n = 4 NB - 3
+    constexpr int Thi = (4 * NB + 1 + 31) / 32;  // n = 4 NB
+    t[NB][Tlo] = k_mfma<NB, Tlo>;
+    if constexpr (Thi != Tlo && Thi <= kMfmaMaxT) t[NB][Thi] = k_mfma<NB, Thi>;
+    if constexpr (NB < kNbHi) MfmaFill<NB + 1>::fill(t);
+  }
+};
+
+#define MCR_CAT2(a, b) a##b
+#define MCR_CAT(a, b) MCR_CAT2(a, b)
+void MCR_CAT(mfma_fill_part, MCR_MFMA_PART)(PathKernel (*t)[kMfmaMaxT + 1]) {
+  MfmaFill<kNbLo>::fill(t);
+}
+
+#if MCR_MFMA_PART == 0
+void mfma_fill_part1(PathKernel (*t)[kMfmaMaxT + 1]);
+void mfma_fill_part2(PathKernel (*t)[kMfmaMaxT + 1]);
+void mfma_fill_part3(PathKernel (*t)[kMfmaMaxT + 1]);
+
+PathKernel mfma_kernel(int n, int T) {
+  static PathKernel table[kMfmaNB + 1][kMfmaMaxT + 1];
+  static bool init = false;
+  if (!init) {
+    for (auto& row : table)
+      for (auto& k : row) k = nullptr;
+    mfma_fill_part0(table);
+    mfma_fill_part1(table);
+    mfma_fill_part2(table);
+    mfma_fill_part3(table);
+    init = true;
+  }
+  const int nb = (n + 3) / 4;
+  if (nb < 1 || nb > kMfmaNB || T < 1 || T > kMfmaMaxT) return nullptr;
+  return table[nb][T];
+}
+#endif
+
+}  // namespace mcr

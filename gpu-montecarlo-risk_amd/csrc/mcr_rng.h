@@ -41,6 +41,15 @@ struct Ctr4 {
   uint32_t x, y, z, w;
 };
 
+// a ^ b ^ c in one v_bitop3_b32 on gfx950 (truth table 0x96); plain xors on the host
+MCR_HD uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+#else
+  return a ^ b ^ c;
+#endif
+}
+
 MCR_HD Ctr4 philox4x32_10(Ctr4 c, uint32_t k0, uint32_t k1) {
 #pragma unroll
   for (int r = 0; r < 10; ++r) {
@@ -51,9 +60,9 @@ MCR_HD Ctr4 philox4x32_10(Ctr4 c, uint32_t k0, uint32_t k1) {
     const uint64_t p0 = (uint64_t)kPhiloxM0 * c.x;
     const uint64_t p1 = (uint64_t)kPhiloxM1 * c.z;
     Ctr4 n;
-    n.x = (uint32_t)(p1 >> 32) ^ c.y ^ k0;
+    n.x = xor3((uint32_t)(p1 >> 32), c.y, k0);
     n.y = (uint32_t)p1;
-    n.z = (uint32_t)(p0 >> 32) ^ c.w ^ k1;
+    n.z = xor3((uint32_t)(p0 >> 32), c.w, k1);
     n.w = (uint32_t)p0;
     c = n;
   }
