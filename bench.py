@@ -23,6 +23,8 @@ PEAK_BF16_TFLOPS = 2500.0   # MI355X published dense bf16 MFMA
 
 WORKLOAD = {"name": "fused_n50_100M", "n": 50, "factors": 5, "days": 2000, "paths": 100_000_000,
             "k": 10, "x": 0.02, "alpha": 0.99, "data_seed": 1002, "path_seed": 20251202}
+LARGE = {"name": "large_n600_10M", "n": 600, "factors": 20, "days": 2500, "paths": 10_000_000,
+         "k": 10, "x": 0.02, "alphas": (0.99, 0.999), "data_seed": 1003, "path_seed": 20251203}
 
 
 def parse():
@@ -33,6 +35,7 @@ def parse():
     p.add_argument("--precision", default="auto", choices=["auto", "fp32", "bf16x3"])
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--cpu-seconds", type=float, default=12.0)
+    p.add_argument("--no-large", action="store_true", help="skip the n=600 leg")
     p.add_argument("--copy-mb", type=int, default=0,
                    help="first run a plain device copy of this many MiB (counter calibration)")
     return p.parse_args()
@@ -61,6 +64,29 @@ def cpu_baseline(L, mu, x, k, seed, n, budget_s):
             "sample": f"{npar} paths of the n=50 workload (OpenMP, {threads} threads); "
                       f"serial 1-core rate {serial_rate:.4g} paths/s on {ns} paths",
             "serial_value": serial_rate}
+
+
+def large_leg(mcr, np):
+    """n = 600 (config 3): plain fp32 tile kernel vs bf16x3 MFMA, same stream."""
+    G = LARGE
+    mu, sigma = mcr.synth_factor_model(G["n"], G["factors"], G["days"], G["data_seed"])
+    res = {"workload": G["name"], "n_assets": G["n"], "paths": G["paths"],
+           "cond_sigma": float(np.linalg.cond(sigma))}
+    for name, prec in (("fp32", mcr.PREC_FP32), ("bf16x3", mcr.PREC_BF16X3)):
+        runs = [mcr.run(mu, sigma, G["x"], G["k"], G["paths"], G["path_seed"], precision=prec,
+                        alphas=G["alphas"]) for _ in range(3)]
+        ms = float(np.mean([r.ms_kernel for r in runs[1:]]))
+        r = runs[-1]
+        res[name] = {"kernel": r.kernel, "kernel_ms": ms, "paths_per_s": G["paths"] / (ms * 1e-3),
+                     "tflops": G["n"] * (G["n"] + 1) * G["paths"] / (ms * 1e-3) / 1e12,
+                     "count": r.count, "p_hat": r.p_hat, "var99": r.var[0], "var999": r.var[1]}
+    f, b = res["fp32"], res["bf16x3"]
+    res["speedup_bf16x3_vs_fp32"] = f["kernel_ms"] / b["kernel_ms"]
+    res["rel_diff_p_hat"] = abs(b["p_hat"] - f["p_hat"]) / f["p_hat"]
+    res["rel_diff_var99"] = abs(b["var99"] - f["var99"]) / f["var99"]
+    print(f"# n=600: fp32 {f['kernel_ms']:.2f} ms, bf16x3 {b['kernel_ms']:.2f} ms", file=sys.stderr,
+          flush=True)
+    return res
 
 
 def main():
@@ -134,6 +160,8 @@ def main():
         "result": {"count": r.count, "p_hat": r.p_hat, "var99": r.var[0], "es99": r.es[0]},
     }
     out.update(extra)
+    if not args.no_large:
+        out["large"] = large_leg(mcr, np)
     if not args.no_cpu:
         L, *_ = mcr.factor(sigma)
         out["cpu_baseline"] = cpu_baseline(L, mu, W["x"], W["k"], W["path_seed"], n,

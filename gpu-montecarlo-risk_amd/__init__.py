@@ -34,9 +34,10 @@ MCR_E_ARG, MCR_E_UNSUPPORTED, MCR_E_DEVICE, MCR_E_NOMEM, MCR_E_FACTOR = -1, -2, 
 MCR_MAX_ALPHA = 4
 MCR_MAX_N = 608
 PREC_FP32, PREC_BF16X3, PREC_AUTO = 0, 1, 2
-KERNEL_AUTO, KERNEL_VALU_REG, KERNEL_VALU_TILE, KERNEL_MFMA = 0, 1, 2, 3
+KERNEL_AUTO, KERNEL_VALU_REG, KERNEL_VALU_TILE, KERNEL_MFMA, KERNEL_MFMA_LDS = 0, 1, 2, 3, 4
 F_VAR, F_NEAR, F_SHIST, F_TEST_NARROW = 0x1, 0x2, 0x4, 0x100
-KERNEL_NAMES = {KERNEL_VALU_REG: "valu_reg", KERNEL_VALU_TILE: "valu_tile", KERNEL_MFMA: "mfma"}
+KERNEL_NAMES = {KERNEL_VALU_REG: "valu_reg", KERNEL_VALU_TILE: "valu_tile", KERNEL_MFMA: "mfma",
+                KERNEL_MFMA_LDS: "mfma_lds"}
 
 
 class McrError(RuntimeError):

@@ -230,15 +230,8 @@ int choose_kernel(const mcr_query* q, Plan* p) {
   if (k == MCR_KERNEL_AUTO) {
     if (q->precision == MCR_PREC_FP32) {
       k = n <= kRegMaxN ? MCR_KERNEL_VALU_REG : MCR_KERNEL_VALU_TILE;
-    } else if (q->precision == MCR_PREC_BF16X3) {
-      if (n > 32 * kMfmaMaxT - 1) {
-        set_error("bf16x3 MFMA kernel supports n <= %d (n = %d)", 32 * kMfmaMaxT - 1, n);
-        return MCR_E_UNSUPPORTED;
-      }
-      k = MCR_KERNEL_MFMA;
-    } else if (q->precision == MCR_PREC_AUTO) {
-      k = n <= 32 * kMfmaMaxT - 1 ? MCR_KERNEL_MFMA
-                                  : (n <= kRegMaxN ? MCR_KERNEL_VALU_REG : MCR_KERNEL_VALU_TILE);
+    } else if (q->precision == MCR_PREC_BF16X3 || q->precision == MCR_PREC_AUTO) {
+      k = n <= 32 * kMfmaMaxT - 1 ? MCR_KERNEL_MFMA : MCR_KERNEL_MFMA_LDS;
     } else {
       set_error("unknown precision %d", q->precision);
       return MCR_E_ARG;
@@ -252,12 +245,12 @@ int choose_kernel(const mcr_query* q, Plan* p) {
     set_error("MFMA kernel supports n <= %d (n = %d)", 32 * kMfmaMaxT - 1, n);
     return MCR_E_UNSUPPORTED;
   }
-  if (k < MCR_KERNEL_VALU_REG || k > MCR_KERNEL_MFMA) {
+  if (k < MCR_KERNEL_VALU_REG || k > MCR_KERNEL_MFMA_LDS) {
     set_error("unknown kernel %d", k);
     return MCR_E_ARG;
   }
   p->kernel = k;
-  if (k == MCR_KERNEL_MFMA) {
+  if (k == MCR_KERNEL_MFMA || k == MCR_KERNEL_MFMA_LDS) {
     p->T = (n + 1 + 31) / 32;
     p->S = (n + 15) / 16;
   } else {

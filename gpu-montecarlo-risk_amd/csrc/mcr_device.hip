@@ -431,6 +431,15 @@ hipError_t launch_paths(int kernel, const PathArgs& a, int num_cus, hipStream_t 
     hipLaunchKernelGGL(k_valu_tile, dim3((unsigned)blocks), dim3(256), lds, stream, a);
     return hipGetLastError();
   }
+  if (kernel == 4) {
+    PathKernel k = mfma_big_kernel(a.T);
+    if (!k || a.n > kMaxN) return hipErrorInvalidValue;
+    // 4-5 row tiles per wave need ~220-255 VGPRs: one or two workgroups per CU
+    const uint64_t per_cu = (a.T + 3) / 4 >= 5 ? 1 : 2;
+    const uint64_t blocks = min_u64((a.npaths + 63) / 64, (uint64_t)num_cus * per_cu);
+    hipLaunchKernelGGL(k, dim3((unsigned)blocks), dim3(256), 0, stream, a);
+    return hipGetLastError();
+  }
   if (kernel == 3) {
     PathKernel k = mfma_kernel(a.n, a.T);
     if (!k) return hipErrorInvalidValue;

@@ -15,32 +15,6 @@ namespace mcr {
 // 8(r/4) + 4(l/32) + r%4; two more swaps bring each path's crash count and
 // loss back to lane l.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t pack_bf16x2(float lo, float hi) {
-  return (uint32_t)bf16_bits(lo) | ((uint32_t)bf16_bits(hi) << 16);
-}
-
-// z[0..7] -> bf16 hi and lo parts, two per 32-bit word
-__device__ __forceinline__ void split8(const float* z, uint32_t hi[4], uint32_t lo[4]) {
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    hi[q] = pack_bf16x2(z[2 * q], z[2 * q + 1]);
-    const float r0 = z[2 * q] - as_f32(hi[q] << 16);
-    const float r1 = z[2 * q + 1] - as_f32(hi[q] & 0xffff0000u);
-    lo[q] = pack_bf16x2(r0, r1);
-  }
-}
-
-// lanes 32-63 of x <-> lanes 0-31 of y
-__device__ __forceinline__ void swap32(uint32_t& x, uint32_t& y) {
-  const auto r = __builtin_amdgcn_permlane32_swap(x, y, false, false);
-  x = r[0];
-  y = r[1];
-}
-
-__device__ __forceinline__ bf16x8 as_bf16x8(const uint32_t w[4]) {
-  return __builtin_bit_cast(bf16x8, make_uint4(w[0], w[1], w[2], w[3]));
-}
-
 // NB = Philox blocks per path (n <= 4 NB), T = row tiles; both compile-time
 // so every block, step and tile bound below is a constant.
 template <int NB, int T>

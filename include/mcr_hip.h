@@ -49,7 +49,8 @@ extern "C" {
 #define MCR_KERNEL_AUTO 0
 #define MCR_KERNEL_VALU_REG 1  /* one path per lane, rows in registers, n <= 64 */
 #define MCR_KERNEL_VALU_TILE 2 /* 64 paths per workgroup, normals in LDS, n <= 608 */
-#define MCR_KERNEL_MFMA 3      /* 32 paths per wave on 32x32x16 bf16 MFMA, n <= 127 */
+#define MCR_KERNEL_MFMA 3      /* 64 paths per wave on 32x32x16 bf16 MFMA, n <= 127 */
+#define MCR_KERNEL_MFMA_LDS 4  /* 64 paths per workgroup, normals staged in LDS, n <= 608 */
 
 /* flags */
 #define MCR_F_VAR 0x1   /* portfolio-loss quantiles (VaR) and tail sums at alpha[] */

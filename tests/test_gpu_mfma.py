@@ -37,6 +37,38 @@ def test_mfma_vs_fp64_same_stream(mcr, orc, n):
         assert abs(r.var[i] - v64) <= 1e-4 * abs(v64), (a, r.var[i], v64)
 
 
+@pytest.mark.parametrize("n,M", [(5, 100_000), (50, 100_000), (128, 40_000), (200, 20_000),
+                                 (333, 10_000), (600, 6_000)])
+def test_mfma_lds_vs_fp64_same_stream(mcr, orc, n, M):
+    """Large-n MFMA kernel (normals staged in LDS), same tolerances."""
+    seed, x = 91, 0.02
+    k = max(1, n // 5)
+    mu, sigma = factor_like(n, min(20, n), 70 + n, T=max(4000, 2 * n))
+    r = mcr.run(mu, sigma, x, k, M, seed, precision=mcr.PREC_BF16X3,
+                kernel=mcr.KERNEL_MFMA_LDS, alphas=(0.99,), want_S=True)
+    assert r.kernel == "mfma_lds"
+    L, *_ = mcr.factor(sigma)
+    w = np.full(n, 1.0 / n)
+    ref = orc.simulate_f64(L, mu, w, x, seed, 0, M, k, want_S=True, want_loss=True, nthreads=8)
+    assert abs(r.count - ref["count"]) <= 3 + 2e-4 * ref["count"], (r.count, ref["count"])
+    assert np.mean(r.S != ref["S"]) <= 2e-3
+    v64 = np.sort(ref["loss"])[rank_of(0.99, M) - 1]
+    assert abs(r.var[0] - v64) <= 1e-4 * abs(v64), (r.var[0], v64)
+
+
+def test_mfma_kernels_agree(mcr):
+    """Small-n MFMA (register operands) and LDS MFMA kernels: same products,
+    summation order may differ in the last fp32 bit, so counts agree to a few
+    paths and VaR to 1e-6."""
+    mu, sigma = factor_like(50, 5, 1002)
+    a = mcr.run(mu, sigma, 0.02, 10, 2_000_000, 3, precision=mcr.PREC_BF16X3,
+                kernel=mcr.KERNEL_MFMA, alphas=(0.99,))
+    b = mcr.run(mu, sigma, 0.02, 10, 2_000_000, 3, precision=mcr.PREC_BF16X3,
+                kernel=mcr.KERNEL_MFMA_LDS, alphas=(0.99,))
+    assert abs(a.count - b.count) <= 5
+    assert abs(a.var[0] - b.var[0]) <= 1e-6 * abs(a.var[0])
+
+
 def test_mfma_independent_streams_agree_with_fp32(mcr):
     mu, sigma = factor_like(50, 5, 1002)
     a = mcr.run(mu, sigma, 0.02, 10, 20_000_000, 1, precision=mcr.PREC_BF16X3)
