@@ -144,6 +144,10 @@ def lib() -> ctypes.CDLL:
         L.hip_simulate.argtypes = [ctypes.POINTER(MonteCarloParams),
                                    ctypes.POINTER(MonteCarloResult)]
         L.get_hip_model.restype = ModelFunctions
+        L.mcr_run_batch.argtypes = [ctypes.POINTER(Query), ctypes.c_int, ctypes.POINTER(Result),
+                                    ctypes.POINTER(BatchOpts)]
+        L.mcr_autotune_batch.argtypes = [ctypes.POINTER(Query), ctypes.c_int,
+                                         ctypes.POINTER(BatchOpts), ctypes.POINTER(ctypes.c_int)]
         _lib = L
     return _lib
 
@@ -190,11 +194,10 @@ class RiskResult:
     s_hist: np.ndarray | None = None
 
 
-def run(mu, sigma, x: float, k: int, paths: int, seed: int = 0, *, weights=None,
-        precision: int = PREC_FP32, kernel: int = KERNEL_AUTO, alphas=(), near: bool = False,
-        want_S: bool = False, want_loss: bool = False, want_hist: bool = False,
-        path_offset: int = 0, extra_flags: int = 0) -> RiskResult:
-    """One risk query on the GPU (mcr_run)."""
+def _make_query(mu, sigma, x, k, paths, seed, weights=None, precision=PREC_FP32,
+                kernel=KERNEL_AUTO, alphas=(), near=False, want_S=False, want_loss=False,
+                want_hist=False, path_offset=0, extra_flags=0):
+    """A filled Query plus the arrays it points into (keep them alive)."""
     mu = np.ascontiguousarray(mu, dtype=np.float64)
     sigma = np.ascontiguousarray(sigma, dtype=np.float64)
     n = int(mu.shape[0])
@@ -229,9 +232,11 @@ def run(mu, sigma, x: float, k: int, paths: int, seed: int = 0, *, weights=None,
     q.S_out = _ptr(S, _i32p)
     q.loss_out = _ptr(loss, _f32p)
     q.s_hist = _ptr(hist, _i64p)
-    r = Result()
-    _check(lib().mcr_run(ctypes.byref(q), ctypes.byref(r)))
-    na = len(alphas)
+    return q, (mu, sigma, w, S, loss, hist)
+
+
+def _result(r: Result, na: int, keep=None) -> RiskResult:
+    S, loss, hist = (keep[3], keep[4], keep[5]) if keep else (None, None, None)
     return RiskResult(count=r.count, near_threshold=r.near_threshold, p_hat=r.p_hat,
                       std_error=r.std_error, ci_lower=r.ci_lower, ci_upper=r.ci_upper,
                       var=list(r.var)[:na], es=list(r.es)[:na],
@@ -240,6 +245,69 @@ def run(mu, sigma, x: float, k: int, paths: int, seed: int = 0, *, weights=None,
                       factor_attempts=r.factor_attempts, factor_clamped=r.factor_clamped,
                       kernel=KERNEL_NAMES.get(r.kernel, str(r.kernel)), ms_kernel=r.ms_kernel,
                       ms_total=r.ms_total, S=S, loss=loss, s_hist=hist)
+
+
+def run(mu, sigma, x: float, k: int, paths: int, seed: int = 0, *, weights=None,
+        precision: int = PREC_FP32, kernel: int = KERNEL_AUTO, alphas=(), near: bool = False,
+        want_S: bool = False, want_loss: bool = False, want_hist: bool = False,
+        path_offset: int = 0, extra_flags: int = 0) -> RiskResult:
+    """One risk query on the GPU (mcr_run)."""
+    q, keep = _make_query(mu, sigma, x, k, paths, seed, weights, precision, kernel, alphas, near,
+                          want_S, want_loss, want_hist, path_offset, extra_flags)
+    r = Result()
+    _check(lib().mcr_run(ctypes.byref(q), ctypes.byref(r)))
+    return _result(r, len(alphas), keep)
+
+
+BATCH_PERSISTENT, BATCH_PER_QUERY = 0, 1
+ORDER_LPT, ORDER_NATURAL = 0, 1
+
+
+class BatchOpts(ctypes.Structure):
+    _fields_ = [("mode", ctypes.c_int), ("use_graph", ctypes.c_int),
+                ("chunk_paths", ctypes.c_int), ("order", ctypes.c_int), ("grid", ctypes.c_int),
+                ("ms_device", ctypes.c_double), ("ms_total", ctypes.c_double),
+                ("chunk_used", ctypes.c_int), ("order_used", ctypes.c_int),
+                ("graph_replayed", ctypes.c_int), ("work_items", ctypes.c_int64)]
+
+
+def _queries(portfolios):
+    """portfolios: dicts with mu, sigma, x, k, paths, seed and optional weights,
+    alphas, precision, kernel, path_offset, extra_flags."""
+    qs = (Query * len(portfolios))()
+    keep = []
+    for i, p in enumerate(portfolios):
+        q, kp = _make_query(p["mu"], p["sigma"], p["x"], p["k"], p["paths"], p.get("seed", 0),
+                            p.get("weights"), p.get("precision", PREC_BF16X3),
+                            p.get("kernel", KERNEL_AUTO), tuple(p.get("alphas", ())),
+                            path_offset=p.get("path_offset", 0),
+                            extra_flags=p.get("extra_flags", 0))
+        qs[i] = q
+        keep.append(kp)
+    return qs, keep
+
+
+def run_batch(portfolios, *, mode: int = BATCH_PERSISTENT, use_graph: bool = True,
+              chunk_paths: int = 0, order: int = ORDER_LPT, grid: int = 0):
+    """Independent queries in one batch (mcr_run_batch): (results, BatchOpts)."""
+    qs, keep = _queries(portfolios)
+    rs = (Result * len(portfolios))()
+    o = BatchOpts(mode, int(use_graph), chunk_paths, order, grid)
+    _check(lib().mcr_run_batch(qs, len(portfolios), rs, ctypes.byref(o)))
+    out = [_result(rs[i], len(portfolios[i].get("alphas", ()))) for i in range(len(portfolios))]
+    del keep
+    return out, o
+
+
+def autotune_batch(portfolios):
+    """Times the persistent batch's candidate settings; the fastest becomes the
+    library default.  Returns (BatchOpts of the winner, candidates timed)."""
+    qs, keep = _queries(portfolios)
+    o = BatchOpts()
+    nc = ctypes.c_int(0)
+    _check(lib().mcr_autotune_batch(qs, len(portfolios), ctypes.byref(o), ctypes.byref(nc)))
+    del keep
+    return o, nc.value
 
 
 def factor(sigma):

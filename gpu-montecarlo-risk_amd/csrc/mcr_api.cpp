@@ -68,6 +68,7 @@ struct Context {
   size_t arena_bytes = 0;
   void* host = nullptr;  // pinned staging
   size_t host_bytes = 0;
+  uint64_t arena_gen = 0;  // bumped whenever the arena moves (invalidates graphs)
 };
 
 Context g_ctx;
@@ -112,6 +113,7 @@ int reserve_arena(size_t bytes) {
     return MCR_E_NOMEM;
   }
   g_ctx.arena_bytes = want;
+  ++g_ctx.arena_gen;
   return MCR_OK;
 }
 
@@ -442,6 +444,7 @@ PathArgs base_args(const mcr_query* q, const Staged& st) {
   a.info = st.d.info;
   a.acc = st.d.acc;
   a.gather = st.d.gather;
+  a.gather_cap = kGatherCap;
   a.hist = st.d.hist;
   return a;
 }
@@ -478,18 +481,18 @@ int run_paths(const mcr_query* q, const Staged& st, PathArgs a, bool per_path, b
 
 int64_t fixed32_host(float v) { return (int64_t)((double)v * 4294967296.0); }
 
-// Resolves quantile al from a bracket pass; returns 1 when resolved.
-int try_resolve(const mcr_query* q, const Staged& st, const DevAccum& acc, int slot, int64_t R,
-                double* var, double* es, int64_t* tail) {
-  const int64_t M = q->paths;
+// Resolves a quantile from a bracket pass (window counts in acc slot `slot`,
+// in-window losses in gather[0 .. min(G, cap))); returns 1 when resolved, 0
+// when the rank missed the window or the buffer overflowed, -1 on copy error.
+int resolve_quantile(int64_t M, const DevAccum& acc, int slot, const float* gather, unsigned cap,
+                     int64_t R, double* var, double* es, int64_t* tail) {
   const int64_t G = (int64_t)acc.gathered[slot];
   const int64_t above = (int64_t)acc.above[slot];
   const int64_t below = M - above - G;
-  if (G > (int64_t)kGatherCap || !(below < R && R <= below + G)) return 0;
+  if (G > (int64_t)cap || !(below < R && R <= below + G)) return 0;
   std::vector<float> g((size_t)G);
   if (G > 0) {
-    if (hipMemcpy(g.data(), st.d.gather + (size_t)slot * kGatherCap, (size_t)G * sizeof(float),
-                  hipMemcpyDeviceToHost) != hipSuccess)
+    if (hipMemcpy(g.data(), gather, (size_t)G * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
       return -1;
   }
   std::sort(g.begin(), g.end());
@@ -500,6 +503,12 @@ int try_resolve(const mcr_query* q, const Staged& st, const DevAccum& acc, int s
   *tail = t;
   *es = (double)t * std::ldexp(1.0, -32) / (double)(M - R + 1);
   return 1;
+}
+
+int try_resolve(const mcr_query* q, const Staged& st, const DevAccum& acc, int slot, int64_t R,
+                double* var, double* es, int64_t* tail) {
+  return resolve_quantile(q->paths, acc, slot, st.d.gather + (size_t)slot * kGatherCap, kGatherCap,
+                          R, var, es, tail);
 }
 
 int upload_windows(const Staged& st, const DevInfo& info) {
@@ -670,6 +679,410 @@ int run_locked(const mcr_query* q, mcr_result* r) {
   return MCR_OK;
 }
 
+
+// ---------------------------------------------------------------------------
+// Batches: batched factor (one workgroup per query), batched prepare, then one
+// persistent launch of k_batch_paths over (query, 64-path group run) work
+// items, optionally captured once in a hipGraph and replayed.
+// ---------------------------------------------------------------------------
+struct Tuned {
+  int chunk_paths = 2048;
+  int order = MCR_ORDER_LPT;
+  bool tuned = false;
+};
+Tuned g_tune;
+
+struct GraphCache {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  uint64_t signature = 0;
+};
+GraphCache g_graph;
+
+void drop_graph() {
+  if (g_graph.exec) (void)hipGraphExecDestroy(g_graph.exec);
+  if (g_graph.graph) (void)hipGraphDestroy(g_graph.graph);
+  g_graph = GraphCache();
+}
+
+uint64_t mix64(uint64_t h, uint64_t v) {
+  h ^= v + 0x9E3779B97F4A7C15ull + (h << 6) + (h >> 2);
+  return h;
+}
+
+struct BatchPlan {
+  int nq = 0, max_n = 0;
+  std::vector<int> T, S, na;
+  std::vector<WorkItem> items;
+  // device regions
+  size_t in_bytes = 0, out_bytes = 0, total = 0;
+  // offsets within the input region
+  std::vector<size_t> offA, offMu, offW;
+  size_t offFa = 0, offPa = 0, offQa = 0, offItems = 0;
+  // offsets of the output region (infos, accs), then scratch
+  size_t offInfo = 0, offAcc = 0;
+  std::vector<size_t> offL, offV, offFrag, offInit, offGather;
+  size_t offNext = 0, offHist = 0;
+};
+
+// Cost of one 64-path group of an n-asset query on the LDS MFMA kernel:
+// S steps of RNG plus sum_t steps(t) tile-steps of MFMA.
+double group_cost(int T, int S) {
+  double tiles = 0.0;
+  for (int t = 0; t < T; ++t) tiles += (t == T - 1) ? S : std::min(2 * t + 2, S);
+  return 4.0 * S + 1.5 * tiles;
+}
+
+int plan_batch(const mcr_query* qs, int nq, int chunk_paths, int order, BatchPlan* bp) {
+  bp->nq = nq;
+  bp->T.resize(nq);
+  bp->S.resize(nq);
+  bp->na.resize(nq);
+  for (int i = 0; i < nq; ++i) {
+    const mcr_query* q = &qs[i];
+    int r = validate(q);
+    if (r != MCR_OK) {
+      char msg[512];
+      snprintf(msg, sizeof(msg), "query %d: %s", i, g_err);
+      set_error("%s", msg);
+      return r;
+    }
+    if (q->precision != MCR_PREC_BF16X3 && q->precision != MCR_PREC_AUTO) {
+      set_error("query %d: persistent batches run the bf16x3 MFMA kernels (precision %d)", i,
+                q->precision);
+      return MCR_E_UNSUPPORTED;
+    }
+    if (q->S_out || q->loss_out || (q->flags & (MCR_F_SHIST | MCR_F_NEAR))) {
+      set_error("query %d: per-path outputs, S histograms and near counts are single-query options",
+                i);
+      return MCR_E_UNSUPPORTED;
+    }
+    bp->T[i] = (q->n + 1 + 31) / 32;
+    bp->S[i] = (q->n + 15) / 16;
+    bp->na[i] = (q->flags & MCR_F_VAR) ? q->n_alpha : 0;
+    bp->max_n = std::max(bp->max_n, q->n);
+  }
+  // work items
+  const uint64_t cg = (uint64_t)std::max(1, chunk_paths / 64);
+  bp->items.clear();
+  for (int i = 0; i < nq; ++i) {
+    const uint64_t G = ((uint64_t)qs[i].paths + 63) / 64;
+    for (uint64_t f = 0; f < G; f += cg) {
+      WorkItem w;
+      w.query = (uint32_t)i;
+      w.first_group = f;
+      w.ngroups = (uint32_t)std::min<uint64_t>(cg, G - f);
+      bp->items.push_back(w);
+    }
+  }
+  if (order == MCR_ORDER_LPT) {
+    std::vector<double> cost(nq);
+    for (int i = 0; i < nq; ++i) cost[i] = group_cost(bp->T[i], bp->S[i]);
+    std::stable_sort(bp->items.begin(), bp->items.end(), [&](const WorkItem& x, const WorkItem& y) {
+      return cost[x.query] * x.ngroups > cost[y.query] * y.ngroups;
+    });
+  }
+  // layout: [inputs | outputs | scratch]
+  Carver c{nullptr};
+  bp->offA.resize(nq);
+  bp->offMu.resize(nq);
+  bp->offW.resize(nq);
+  for (int i = 0; i < nq; ++i) {
+    const size_t n = (size_t)qs[i].n;
+    bp->offA[i] = c.off;
+    c.take<double>(n * n);
+    bp->offMu[i] = c.off;
+    c.take<double>(n);
+    bp->offW[i] = c.off;
+    c.take<double>(n);
+  }
+  bp->offFa = c.off;
+  c.take<FactorArgs>(nq);
+  bp->offPa = c.off;
+  c.take<PrepArgs>(nq);
+  bp->offQa = c.off;
+  c.take<PathArgs>(nq);
+  bp->offItems = c.off;
+  c.take<WorkItem>(bp->items.size());
+  bp->in_bytes = c.off;
+  bp->offInfo = c.off;
+  c.take<DevInfo>(nq);
+  bp->offAcc = c.off;
+  c.take<DevAccum>(nq);
+  bp->out_bytes = c.off - bp->in_bytes;
+  bp->offL.resize(nq);
+  bp->offV.resize(nq);
+  bp->offFrag.resize(nq);
+  bp->offInit.resize(nq);
+  bp->offGather.resize(nq);
+  for (int i = 0; i < nq; ++i) {
+    const size_t n = (size_t)qs[i].n;
+    bp->offL[i] = c.off;
+    c.take<double>(n * n);
+    bp->offV[i] = c.off;
+    c.take<double>(n);
+    bp->offFrag[i] = c.off;
+    c.take<uint4>((size_t)bp->T[i] * bp->S[i] * 128);
+    bp->offInit[i] = c.off;
+    c.take<float>((size_t)32 * bp->T[i]);
+    bp->offGather[i] = c.off;
+    c.take<float>((size_t)std::max(bp->na[i], 1) * kBatchGatherCap);
+  }
+  bp->offNext = c.off;
+  c.take<unsigned>(1);
+  bp->offHist = c.off;
+  c.take<unsigned>(kHistBins);
+  bp->total = c.off;
+  return MCR_OK;
+}
+
+// Fills the pinned image of the input region (host) with inputs and args.
+void fill_batch_inputs(const mcr_query* qs, const BatchPlan& bp, char* h, char* dev) {
+  const int nq = bp.nq;
+  FactorArgs* fa = (FactorArgs*)(h + bp.offFa);
+  PrepArgs* pa = (PrepArgs*)(h + bp.offPa);
+  PathArgs* qa = (PathArgs*)(h + bp.offQa);
+  for (int i = 0; i < nq; ++i) {
+    const mcr_query* q = &qs[i];
+    const size_t n = (size_t)q->n;
+    double* A = (double*)(h + bp.offA[i]);
+    double* mu = (double*)(h + bp.offMu[i]);
+    double* w = (double*)(h + bp.offW[i]);
+    for (size_t r = 0; r < n; ++r) {
+      std::memcpy(A + r * n, q->sigma + r * (size_t)q->ld_sigma, n * sizeof(double));
+      mu[r] = q->mu[r];
+      w[r] = q->weights ? q->weights[r] : 1.0 / (double)n;
+    }
+    DevInfo* info = (DevInfo*)(dev + bp.offInfo) + i;
+    DevAccum* acc = (DevAccum*)(dev + bp.offAcc) + i;
+    FactorArgs f = {};
+    f.A = (const double*)(dev + bp.offA[i]);
+    f.mu = (const double*)(dev + bp.offMu[i]);
+    f.w = (const double*)(dev + bp.offW[i]);
+    f.L = (double*)(dev + bp.offL[i]);
+    f.v64 = (double*)(dev + bp.offV[i]);
+    f.n = q->n;
+    f.n_alpha = bp.na[i];
+    for (int al = 0; al < f.n_alpha; ++al) {
+      const double av = q->alpha[al];
+      const double za = inv_norm_cdf(av);
+      const double dens = std::exp(-0.5 * za * za) / std::sqrt(2.0 * M_PI);
+      f.za[al] = za;
+      f.ha[al] = (q->flags & MCR_F_TEST_NARROW)
+                     ? 0.0
+                     : 6.0 * std::sqrt(av * (1.0 - av) / (double)q->paths) / dens + 1e-4;
+    }
+    f.info = info;
+    f.acc = acc;
+    f.hist = (unsigned*)(dev + bp.offHist);
+    fa[i] = f;
+    PrepArgs pr = {};
+    pr.L = f.L;
+    pr.mu = f.mu;
+    pr.n = q->n;
+    pr.T = bp.T[i];
+    pr.S = bp.S[i];
+    pr.info = info;
+    pr.v64 = f.v64;
+    pr.lfrag = (uint4*)(dev + bp.offFrag[i]);
+    pr.rowinit = (float*)(dev + bp.offInit[i]);
+    pa[i] = pr;
+    PathArgs pp = {};
+    pp.seed = q->seed;
+    pp.path0 = q->path_offset;
+    pp.npaths = (uint64_t)q->paths;
+    pp.n = q->n;
+    pp.k = q->k;
+    pp.thr = (float)(-q->x);
+    pp.thr_lo = 1.0f;
+    pp.thr_hi = 0.0f;
+    pp.lfrag = pr.lfrag;
+    pp.rowinit = pr.rowinit;
+    pp.T = pr.T;
+    pp.S = pr.S;
+    pp.info = info;
+    pp.acc = acc;
+    pp.loss_mode = bp.na[i] ? kLossBracket : kLossNone;
+    pp.n_alpha = bp.na[i];
+    pp.gather = (float*)(dev + bp.offGather[i]);
+    pp.gather_cap = kBatchGatherCap;
+    pp.hist = f.hist;
+    qa[i] = pp;
+  }
+  std::memcpy(h + bp.offItems, bp.items.data(), bp.items.size() * sizeof(WorkItem));
+}
+
+uint64_t batch_signature(const mcr_query* qs, const BatchPlan& bp, int grid, int chunk, int order) {
+  uint64_t h = mix64(0, (uint64_t)(uintptr_t)g_ctx.arena);
+  h = mix64(h, g_ctx.arena_gen);
+  h = mix64(h, (uint64_t)(uintptr_t)g_ctx.host);  // the captured copy targets it
+  h = mix64(h, (uint64_t)bp.nq);
+  h = mix64(h, (uint64_t)grid);
+  h = mix64(h, (uint64_t)chunk);
+  h = mix64(h, (uint64_t)order);
+  h = mix64(h, bp.total);
+  for (int i = 0; i < bp.nq; ++i) {
+    h = mix64(h, (uint64_t)qs[i].n);
+    h = mix64(h, (uint64_t)qs[i].paths);
+    h = mix64(h, (uint64_t)bp.na[i]);
+  }
+  return h;
+}
+
+// Device work of a batch on g_ctx.stream (the part a graph captures).
+int enqueue_batch(const BatchPlan& bp, char* dev, char* hout, int grid) {
+  const FactorArgs* fa = (const FactorArgs*)(dev + bp.offFa);
+  const PrepArgs* pa = (const PrepArgs*)(dev + bp.offPa);
+  BatchArgs b;
+  b.qs = (const PathArgs*)(dev + bp.offQa);
+  b.items = (const WorkItem*)(dev + bp.offItems);
+  b.nitems = (uint32_t)bp.items.size();
+  b.next = (unsigned*)(dev + bp.offNext);
+  HIP_TRY(launch_factor_batch(fa, bp.nq, g_ctx.stream));
+  HIP_TRY(launch_prep_batch(pa, bp.nq, bp.max_n, g_ctx.stream));
+  HIP_TRY(launch_batch_clear(b, g_ctx.stream));
+  HIP_TRY(launch_batch_paths(b, grid, g_ctx.stream));
+  HIP_TRY(hipMemcpyAsync(hout, dev + bp.offInfo, bp.out_bytes, hipMemcpyDeviceToHost, g_ctx.stream));
+  return MCR_OK;
+}
+
+int run_batch_locked(const mcr_query* qs, int nq, mcr_result* rs, mcr_batch_opts* o) {
+  const auto t0 = std::chrono::steady_clock::now();
+  if (!qs || !rs || nq < 1) {
+    set_error("batch needs queries, results and nq >= 1");
+    return MCR_E_ARG;
+  }
+  MCR_TRY(ensure_init(g_ctx.device));
+  HIP_TRY(hipSetDevice(g_ctx.device));
+  if (o->mode == MCR_BATCH_PER_QUERY) {
+    double dev_ms = 0.0;
+    for (int i = 0; i < nq; ++i) {
+      int r = run_locked(&qs[i], &rs[i]);
+      if (r != MCR_OK) return r;
+      dev_ms += rs[i].ms_kernel;
+    }
+    o->ms_device = dev_ms;
+    o->ms_total =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return MCR_OK;
+  }
+  if (o->mode != MCR_BATCH_PERSISTENT) {
+    set_error("unknown batch mode %d", o->mode);
+    return MCR_E_ARG;
+  }
+  const int chunk = o->chunk_paths > 0 ? o->chunk_paths : g_tune.chunk_paths;
+  const int order = o->chunk_paths > 0 ? o->order : g_tune.order;
+  if (chunk % 64 != 0) {
+    set_error("chunk_paths = %d is not a multiple of 64", chunk);
+    return MCR_E_ARG;
+  }
+  BatchPlan bp;
+  MCR_TRY(plan_batch(qs, nq, chunk, order, &bp));
+  const int grid = o->grid > 0 ? o->grid : g_ctx.num_cus;
+  MCR_TRY(reserve_arena(bp.total));
+  MCR_TRY(reserve_host(bp.in_bytes + bp.out_bytes));
+  char* dev = (char*)g_ctx.arena;
+  char* hin = (char*)g_ctx.host;
+  char* hout = hin + bp.in_bytes;
+  fill_batch_inputs(qs, bp, hin, dev);
+  HIP_TRY(hipMemcpyAsync(dev, hin, bp.in_bytes, hipMemcpyHostToDevice, g_ctx.stream));
+  HIP_TRY(hipEventRecord(g_ctx.ev0, g_ctx.stream));
+  o->graph_replayed = 0;
+  if (o->use_graph) {
+    const uint64_t sig = batch_signature(qs, bp, grid, chunk, order);
+    if (!g_graph.exec || g_graph.signature != sig) {
+      drop_graph();
+      HIP_TRY(hipStreamBeginCapture(g_ctx.stream, hipStreamCaptureModeThreadLocal));
+      const int r = enqueue_batch(bp, dev, hout, grid);
+      hipGraph_t graph = nullptr;
+      const hipError_t e = hipStreamEndCapture(g_ctx.stream, &graph);
+      if (r != MCR_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return r;
+      }
+      HIP_TRY(e);
+      g_graph.graph = graph;
+      HIP_TRY(hipGraphInstantiate(&g_graph.exec, graph, nullptr, nullptr, 0));
+      g_graph.signature = sig;
+    } else {
+      o->graph_replayed = 1;
+    }
+    HIP_TRY(hipGraphLaunch(g_graph.exec, g_ctx.stream));
+  } else {
+    MCR_TRY(enqueue_batch(bp, dev, hout, grid));
+  }
+  HIP_TRY(hipEventRecord(g_ctx.ev1, g_ctx.stream));
+  HIP_TRY(hipEventSynchronize(g_ctx.ev1));
+  float ms = 0.0f;
+  HIP_TRY(hipEventElapsedTime(&ms, g_ctx.ev0, g_ctx.ev1));
+  o->ms_device = ms;
+  o->chunk_used = chunk;
+  o->order_used = order;
+  o->work_items = (int64_t)bp.items.size();
+
+  const DevInfo* infos = (const DevInfo*)hout;
+  const DevAccum* accs = (const DevAccum*)(hout + (bp.offAcc - bp.offInfo));
+  std::vector<int> refine;
+  for (int i = 0; i < nq; ++i) {
+    const mcr_query* q = &qs[i];
+    mcr_result* r = &rs[i];
+    std::memset(r, 0, sizeof(*r));
+    const int fr = check_factor(infos[i]);
+    if (fr != MCR_OK) {
+      char msg[512];
+      snprintf(msg, sizeof(msg), "query %d: %s", i, g_err);
+      set_error("%s", msg);
+      return fr;
+    }
+    const int64_t M = q->paths;
+    r->count = (int64_t)accs[i].count;
+    r->p_hat = (double)r->count / (double)M;
+    r->std_error = std::sqrt(r->p_hat * (1.0 - r->p_hat) / (double)M);
+    r->ci_lower = std::fmax(0.0, r->p_hat - 1.96 * r->std_error);
+    r->ci_upper = std::fmin(1.0, r->p_hat + 1.96 * r->std_error);
+    r->loss_mean = -infos[i].mp;
+    r->loss_sd = infos[i].loss_sd;
+    r->jitter = infos[i].jitter;
+    r->factor_attempts = infos[i].attempts;
+    r->factor_clamped = infos[i].clamped;
+    r->kernel = MCR_KERNEL_MFMA_LDS;
+    r->ms_kernel = ms;
+    r->var_passes = bp.na[i] ? 1 : 0;
+    bool ok = true;
+    for (int al = 0; al < bp.na[i] && ok; ++al) {
+      const int64_t R = rank_of(q->alpha[al], M);
+      const float* gd = (const float*)(dev + bp.offGather[i]) + (size_t)al * kBatchGatherCap;
+      const int res = resolve_quantile(M, accs[i], al, gd, kBatchGatherCap, R, &r->var[al],
+                                       &r->es[al], &r->tail_sum_fixed[al]);
+      if (res < 0) {
+        set_error("copy of gathered losses failed");
+        return MCR_E_DEVICE;
+      }
+      if (res == 0) ok = false;
+    }
+    if (!ok) refine.push_back(i);
+  }
+  // Rare: a quantile missed its window.  Recompute that query alone on the
+  // same kernel family (identical per-path losses) with the refinement passes.
+  for (int i : refine) {
+    mcr_query q = qs[i];
+    q.kernel = MCR_KERNEL_MFMA_LDS;
+    q.precision = MCR_PREC_BF16X3;
+    mcr_result one;
+    MCR_TRY(run_locked(&q, &one));
+    for (int al = 0; al < bp.na[i]; ++al) {
+      rs[i].var[al] = one.var[al];
+      rs[i].es[al] = one.es[al];
+      rs[i].tail_sum_fixed[al] = one.tail_sum_fixed[al];
+    }
+    rs[i].var_passes = 1 + one.var_passes;
+  }
+  o->ms_total =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  for (int i = 0; i < nq; ++i) rs[i].ms_total = o->ms_total;
+  return MCR_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -691,6 +1104,7 @@ int mcr_shutdown(void) {
   std::lock_guard<std::mutex> lk(g_mu);
   if (!g_ctx.ready) return MCR_OK;
   HIP_TRY(hipStreamSynchronize(g_ctx.stream));
+  drop_graph();
   if (g_ctx.arena) HIP_TRY(hipFree(g_ctx.arena));
   if (g_ctx.host) HIP_TRY(hipHostFree(g_ctx.host));
   HIP_TRY(hipEventDestroy(g_ctx.ev0));
@@ -793,6 +1207,44 @@ int mcr_normals(uint64_t seed, uint64_t path0, int64_t npaths, int n, float* out
   HIP_TRY(launch_normals(seed, path0, (uint64_t)npaths, n, (float*)g_ctx.arena, g_ctx.stream));
   HIP_TRY(hipMemcpyAsync(out, g_ctx.arena, bytes, hipMemcpyDeviceToHost, g_ctx.stream));
   HIP_TRY(hipStreamSynchronize(g_ctx.stream));
+  return MCR_OK;
+}
+
+
+int mcr_run_batch(const mcr_query* qs, int nq, mcr_result* rs, mcr_batch_opts* opts) {
+  mcr_batch_opts local = {};
+  mcr_batch_opts* o = opts ? opts : &local;
+  std::lock_guard<std::mutex> lk(g_mu);
+  return run_batch_locked(qs, nq, rs, o);
+}
+
+int mcr_autotune_batch(const mcr_query* qs, int nq, mcr_batch_opts* best, int* n_candidates) {
+  static const int kChunks[] = {512, 2048, 8192, 32768};
+  static const int kOrders[] = {MCR_ORDER_LPT, MCR_ORDER_NATURAL};
+  std::lock_guard<std::mutex> lk(g_mu);
+  std::vector<mcr_result> rs((size_t)std::max(nq, 1));
+  double best_ms = 1e300;
+  mcr_batch_opts pick = {};
+  int tried = 0;
+  for (int c : kChunks) {
+    for (int od : kOrders) {
+      mcr_batch_opts o = {};
+      o.mode = MCR_BATCH_PERSISTENT;
+      o.chunk_paths = c;
+      o.order = od;
+      MCR_TRY(run_batch_locked(qs, nq, rs.data(), &o));
+      ++tried;
+      if (o.ms_device < best_ms) {
+        best_ms = o.ms_device;
+        pick = o;
+      }
+    }
+  }
+  g_tune.chunk_paths = pick.chunk_used;
+  g_tune.order = pick.order_used;
+  g_tune.tuned = true;
+  if (best) *best = pick;
+  if (n_candidates) *n_candidates = tried;
   return MCR_OK;
 }
 

@@ -43,9 +43,7 @@ __device__ double block_sum_f64(double v, double* red) {
   return s;
 }
 
-__global__ __launch_bounds__(1024) void k_factor(FactorArgs a) {
-  __shared__ double col[kMaxN];
-  __shared__ double red[16];
+__device__ void factor_body(const FactorArgs& a, double* col, double* red) {
   const int n = a.n;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const double* __restrict__ A = a.A;
@@ -155,35 +153,54 @@ __global__ __launch_bounds__(1024) void k_factor(FactorArgs a) {
   }
 }
 
+__global__ __launch_bounds__(1024) void k_factor(FactorArgs a) {
+  __shared__ double col[kMaxN];
+  __shared__ double red[16];
+  factor_body(a, col, red);
+}
+
+// One workgroup per query of a batch.
+__global__ __launch_bounds__(1024) void k_factor_batch(const FactorArgs* fas) {
+  __shared__ double col[kMaxN];
+  __shared__ double red[16];
+  const FactorArgs a = fas[blockIdx.x];
+  factor_body(a, col, red);
+}
+
 hipError_t launch_factor(const FactorArgs& a, hipStream_t stream) {
   hipLaunchKernelGGL(k_factor, dim3(1), dim3(1024), 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_factor_batch(const FactorArgs* fas, int nq, hipStream_t stream) {
+  hipLaunchKernelGGL(k_factor_batch, dim3(nq), dim3(1024), 0, stream, fas);
   return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
 // Prep: layouts of the factor for the three path kernels.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
+__device__ void prep_body(const PrepArgs& a, size_t gid, size_t gsz) {
   const int n = a.n;
-  const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t gsz = (size_t)gridDim.x * blockDim.x;
   const double* __restrict__ L = a.L;
   const int r8 = ((n + kTileRows - 1) / kTileRows) * kTileRows;
-  // fp32 loading and mean
-  for (size_t e = gid; e < (size_t)n; e += gsz) {
-    a.v32[e] = (float)a.v64[e];
-    a.mu32[e] = (float)a.mu[e];
-    a.colpack[colpack_offset(n, (int)e) + n - (int)e] = (float)a.v64[e];
-  }
-  // register kernel: column-packed lower triangle
-  for (size_t e = gid; e < (size_t)n * n; e += gsz) {
-    const int i = (int)(e / n), j = (int)(e % n);
-    if (j <= i) a.colpack[colpack_offset(n, j) + i - j] = (float)L[e];
-  }
-  // tile kernel: zero-padded r8 x r8 row-major
-  for (size_t e = gid; e < (size_t)r8 * r8; e += gsz) {
-    const int i = (int)(e / r8), j = (int)(e % r8);
-    a.L32[e] = (i < n && j <= i) ? (float)L[(size_t)i * n + j] : 0.0f;
+  if (a.colpack) {
+    // fp32 loading and mean
+    for (size_t e = gid; e < (size_t)n; e += gsz) {
+      a.v32[e] = (float)a.v64[e];
+      a.mu32[e] = (float)a.mu[e];
+      a.colpack[colpack_offset(n, (int)e) + n - (int)e] = (float)a.v64[e];
+    }
+    // register kernel: column-packed lower triangle
+    for (size_t e = gid; e < (size_t)n * n; e += gsz) {
+      const int i = (int)(e / n), j = (int)(e % n);
+      if (j <= i) a.colpack[colpack_offset(n, j) + i - j] = (float)L[e];
+    }
+    // tile kernel: zero-padded r8 x r8 row-major
+    for (size_t e = gid; e < (size_t)r8 * r8; e += gsz) {
+      const int i = (int)(e / r8), j = (int)(e % r8);
+      a.L32[e] = (i < n && j <= i) ? (float)L[(size_t)i * n + j] : 0.0f;
+    }
   }
   // MFMA: augmented rows (assets, zero pads, loss row 32T-1 = -v), hi/lo bf16
   if (a.T > 0) {
@@ -226,6 +243,24 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
       a.rowinit[e] = row < n ? (float)a.mu[row] : (row == 32 * T - 1 ? -mp32 : INFINITY);
     }
   }
+}
+
+__global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
+  prep_body(a, (size_t)blockIdx.x * blockDim.x + threadIdx.x, (size_t)gridDim.x * blockDim.x);
+}
+
+// blockIdx.y = query of a batch
+__global__ __launch_bounds__(256) void k_prep_batch(const PrepArgs* pas) {
+  const PrepArgs a = pas[blockIdx.y];
+  prep_body(a, (size_t)blockIdx.x * blockDim.x + threadIdx.x, (size_t)gridDim.x * blockDim.x);
+}
+
+hipError_t launch_prep_batch(const PrepArgs* pas, int nq, int max_n, hipStream_t stream) {
+  int blocks = (max_n * max_n + 255) / 256;
+  if (blocks > 256) blocks = 256;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(k_prep_batch, dim3(blocks, nq), dim3(256), 0, stream, pas);
+  return hipGetLastError();
 }
 
 hipError_t launch_prep(const PrepArgs& a, hipStream_t stream) {

@@ -14,6 +14,7 @@ constexpr int kMfmaNB = 32;     // MFMA kernel: Philox blocks per path (n <= 127
 constexpr int kTilePaths = 64;  // paths per workgroup in the tile kernel
 constexpr int kTileRows = 8;    // rows per register block in the tile kernel
 constexpr unsigned kGatherCap = 1u << 18;  // loss values kept per VaR window
+constexpr unsigned kBatchGatherCap = 1u << 16;  // per window, per query of a batch
 constexpr int kHistBins = 4096;            // refinement histogram
 
 enum LossMode : int { kLossNone = 0, kLossBracket = 1, kLossHist = 2 };
@@ -67,7 +68,8 @@ struct PathArgs {
   unsigned long long* s_hist;  // optional [n+1]
   int loss_mode;
   int n_alpha;
-  float* gather;         // [n_alpha][kGatherCap]
+  float* gather;         // [n_alpha][gather_cap]
+  unsigned gather_cap;   // losses kept per window
   unsigned* hist;        // [kHistBins] refinement
 };
 
@@ -104,6 +106,25 @@ struct PrepArgs {
 // Launchers (mcr_device.hip).  All asynchronous on `stream`.
 hipError_t launch_factor(const FactorArgs& a, hipStream_t stream);
 hipError_t launch_prep(const PrepArgs& a, hipStream_t stream);
+hipError_t launch_factor_batch(const FactorArgs* fas, int nq, hipStream_t stream);
+hipError_t launch_prep_batch(const PrepArgs* pas, int nq, int max_n, hipStream_t stream);
+
+// Batched persistent execution: work item = a run of 64-path groups of one query.
+struct WorkItem {
+  uint32_t query;
+  uint32_t ngroups;
+  uint64_t first_group;
+};
+
+struct BatchArgs {
+  const PathArgs* qs;       // [nq] per-query kernel arguments (device)
+  const WorkItem* items;    // [nitems] in dispatch order
+  uint32_t nitems;
+  unsigned* next;           // work counter, cleared by k_batch_clear
+};
+
+hipError_t launch_batch_clear(const BatchArgs& b, hipStream_t stream);
+hipError_t launch_batch_paths(const BatchArgs& b, int grid, hipStream_t stream);
 hipError_t launch_paths(int kernel, const PathArgs& a, int num_cus, hipStream_t stream);
 hipError_t launch_normals(uint64_t seed, uint64_t path0, uint64_t npaths, int n, float* out,
                           hipStream_t stream);

@@ -21,6 +21,18 @@ __device__ __forceinline__ MCR_CONSTP(T) cview(const T* p) {
   return (MCR_CONSTP(T))p;
 }
 
+// A read-only struct loaded word by word through the constant address space
+// (scalar loads when the address is uniform).
+template <typename T>
+__device__ __forceinline__ T load_const(const T* p) {
+  static_assert(sizeof(T) % 4 == 0, "word-sized struct expected");
+  uint32_t w[sizeof(T) / 4];
+  MCR_CONSTP(uint32_t) src = cview(reinterpret_cast<const uint32_t*>(p));
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(T) / 4); ++i) w[i] = src[i];
+  return __builtin_bit_cast(T, w);
+}
+
 __device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
@@ -62,7 +74,7 @@ __device__ __forceinline__ void append_loss(const PathArgs& a, int al, bool in, 
   base = __shfl(base, leader);
   if (in) {
     const unsigned long long idx = base + rank;
-    if (idx < kGatherCap) a.gather[(size_t)al * kGatherCap + idx] = v;
+    if (idx < a.gather_cap) a.gather[(size_t)al * a.gather_cap + idx] = v;
   }
 }
 

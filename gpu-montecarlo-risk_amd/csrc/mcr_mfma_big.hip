@@ -43,19 +43,20 @@ __device__ __forceinline__ void gen_step(ZTile* zt, int buf, int s, int nb, Key2
   *lo = make_uint2(l0, l1);
 }
 
+struct BigShared {
+  ZTile zt;
+  uint32_t sS[4][64];
+  float sLoss[64];
+};
+
+// Groups g0, g0 + gstep, ... < g1 of 64 paths of query a, accumulated into t.
 template <int TW>
-__global__ __launch_bounds__(256) void k_mfma_big(PathArgs a) {
-  __shared__ ZTile zt;
-  __shared__ uint32_t sS[4][64];
-  __shared__ float sLoss[64];
-  __shared__ unsigned sh_hist[kMaxN + 1];
+__device__ __forceinline__ void mfma_lds_groups(const PathArgs& a, uint64_t g0, uint64_t g1,
+                                                uint64_t gstep, BigShared& sh, LaneTally& t,
+                                                unsigned* sh_hist) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 5;
   const int n = a.n, T = a.T, S = a.S;
   const int nb = (n + 3) >> 2;
-  const bool want_hist = a.s_hist != nullptr;
-  if (want_hist) {
-    for (int i = threadIdx.x; i <= n; i += blockDim.x) sh_hist[i] = 0u;
-  }
   const Key2 key = key_of(a.seed);
   const uint4* __restrict__ lf = a.lfrag + lane * 2;
   const float4* __restrict__ rinit = (const float4*)a.rowinit;
@@ -69,21 +70,19 @@ __global__ __launch_bounds__(256) void k_mfma_big(PathArgs a) {
   const int loss_slot = (T - 1) / 4;  // i of tile T-1 (its owner wave)
   const int loss_wave = (loss_slot & 1) ? 3 - (T - 1) % 4 : (T - 1) % 4;
 
-  LaneTally t;
-  zero_tally(t);
   const int rp = threadIdx.x & 63, ru = threadIdx.x >> 6;
-  for (uint64_t grp = blockIdx.x; grp * 64 < a.npaths; grp += gridDim.x) {
+  for (uint64_t grp = g0; grp < g1; grp += gstep) {
     const uint64_t rpath = a.path0 + grp * 64 + rp;
-    gen_step(&zt, 0, 0, nb, key, rpath, rp, ru);
+    gen_step(&sh.zt, 0, 0, nb, key, rpath, rp, ru);
     __syncthreads();
     f32x16 accA[TW], accB[TW];
     for (int s = 0; s < S; ++s) {
-      if (s + 1 < S) gen_step(&zt, (s + 1) & 1, s + 1, nb, key, rpath, rp, ru);
+      if (s + 1 < S) gen_step(&sh.zt, (s + 1) & 1, s + 1, nb, key, rpath, rp, ru);
       const int cb = s & 1;
-      const bf16x8 ah = __builtin_bit_cast(bf16x8, zt.v[cb][0][0][lane]);
-      const bf16x8 al = __builtin_bit_cast(bf16x8, zt.v[cb][0][1][lane]);
-      const bf16x8 bh = __builtin_bit_cast(bf16x8, zt.v[cb][1][0][lane]);
-      const bf16x8 bl = __builtin_bit_cast(bf16x8, zt.v[cb][1][1][lane]);
+      const bf16x8 ah = __builtin_bit_cast(bf16x8, sh.zt.v[cb][0][0][lane]);
+      const bf16x8 al = __builtin_bit_cast(bf16x8, sh.zt.v[cb][0][1][lane]);
+      const bf16x8 bh = __builtin_bit_cast(bf16x8, sh.zt.v[cb][1][0][lane]);
+      const bf16x8 bl = __builtin_bit_cast(bf16x8, sh.zt.v[cb][1][1][lane]);
 #pragma unroll
       for (int i = 0; i < TW; ++i) {
         if (s < steps[i]) {
@@ -129,7 +128,7 @@ __global__ __launch_bounds__(256) void k_mfma_big(PathArgs a) {
       }
     }
     swap32(SA, SB);
-    sS[wave][lane] = SA + SB;
+    sh.sS[wave][lane] = SA + SB;
     if (wave == loss_wave) {
       float la = 0.0f, lb = 0.0f;
 #pragma unroll
@@ -140,22 +139,76 @@ __global__ __launch_bounds__(256) void k_mfma_big(PathArgs a) {
         }
       uint32_t ua = as_u32(la), ub = as_u32(lb);
       swap32(ua, ub);
-      sLoss[lane] = as_f32(ub);
+      sh.sLoss[lane] = as_f32(ub);
     }
     __syncthreads();
     const uint64_t q = grp * 64 + lane;
     if (wave == 0 && q < a.npaths) {
-      const int Sp = (int)(sS[0][lane] + sS[1][lane] + sS[2][lane] + sS[3][lane]);
-      tally_path(a, t, q, Sp, sLoss[lane], false, want_hist ? sh_hist : nullptr);
+      const int Sp = (int)(sh.sS[0][lane] + sh.sS[1][lane] + sh.sS[2][lane] + sh.sS[3][lane]);
+      tally_path(a, t, q, Sp, sh.sLoss[lane], false, sh_hist);
     }
     __syncthreads();
   }
+}
+
+template <int TW>
+__global__ __launch_bounds__(256) void k_mfma_big(PathArgs a) {
+  __shared__ BigShared sh;
+  __shared__ unsigned sh_hist[kMaxN + 1];
+  const bool want_hist = a.s_hist != nullptr;
+  if (want_hist) {
+    for (int i = threadIdx.x; i <= a.n; i += blockDim.x) sh_hist[i] = 0u;
+  }
+  LaneTally t;
+  zero_tally(t);
+  mfma_lds_groups<TW>(a, blockIdx.x, (a.npaths + 63) / 64, gridDim.x, sh, t,
+                      want_hist ? sh_hist : nullptr);
   flush_tally(a, t);
   if (want_hist) {
     __syncthreads();
-    for (int i = threadIdx.x; i <= n; i += blockDim.x)
+    for (int i = threadIdx.x; i <= a.n; i += blockDim.x)
       if (sh_hist[i]) atomicAdd(&a.s_hist[i], (unsigned long long)sh_hist[i]);
   }
+}
+
+// Persistent batch kernel: every workgroup takes work items (a run of 64-path
+// groups of one query) from an atomic counter until the list is exhausted;
+// the exit test is uniform, so every wave of every workgroup leaves the loop.
+// Tallies are flushed into the item's query after each item.
+constexpr int kBatchTW = (kMaxN + 1 + 127) / 128;
+
+__global__ __launch_bounds__(256) void k_batch_paths(BatchArgs b) {
+  __shared__ BigShared sh;
+  __shared__ unsigned s_item;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = atomicAdd(b.next, 1u);
+    __syncthreads();
+    // uniform by construction; readfirstlane lets the descriptor live in SGPRs
+    const unsigned it = __builtin_amdgcn_readfirstlane(s_item);
+    __syncthreads();
+    if (it >= b.nitems) break;
+    // item and descriptor are read-only for the launch: scalar loads
+    const WorkItem w = load_const(b.items + it);
+    const PathArgs a = load_const(b.qs + w.query);
+    LaneTally t;
+    zero_tally(t);
+    mfma_lds_groups<kBatchTW>(a, w.first_group, w.first_group + w.ngroups, 1, sh, t, nullptr);
+    flush_tally(a, t);
+  }
+}
+
+__global__ void k_batch_clear(BatchArgs b) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *b.next = 0u;
+}
+
+hipError_t launch_batch_clear(const BatchArgs& b, hipStream_t stream) {
+  hipLaunchKernelGGL(k_batch_clear, dim3(1), dim3(64), 0, stream, b);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_paths(const BatchArgs& b, int grid, hipStream_t stream) {
+  hipLaunchKernelGGL(k_batch_paths, dim3(grid), dim3(256), 0, stream, b);
+  return hipGetLastError();
 }
 
 PathKernel mfma_big_kernel(int T) {

@@ -135,6 +135,39 @@ int mcr_prepare(const mcr_query *q, mcr_model_export *out);
 /* The device normal stream: out[p*n + j] = z_j of path path0 + p. */
 int mcr_normals(uint64_t seed, uint64_t path0, int64_t npaths, int n, float *out);
 
+/* ---- batches of independent queries (portfolios) ---- */
+#define MCR_BATCH_PERSISTENT 0 /* batched factor + prepare, then ONE persistent path launch */
+#define MCR_BATCH_PER_QUERY 1  /* one mcr_run per query (the comparison baseline) */
+#define MCR_ORDER_LPT 0        /* work items by descending cost (longest first) */
+#define MCR_ORDER_NATURAL 1    /* work items in query order */
+
+typedef struct mcr_batch_opts {
+  int mode;          /* MCR_BATCH_* */
+  int use_graph;     /* capture the device work in a hipGraph (replayed while the batch
+                        layout is unchanged) */
+  int chunk_paths;   /* paths per work item, multiple of 64; 0 = autotuned value */
+  int order;         /* MCR_ORDER_*; used when chunk_paths != 0 */
+  int grid;          /* persistent workgroups; 0 = one per CU */
+  double ms_device;  /* out: device time of the batch (HIP events) */
+  double ms_total;   /* out: host wall time of the call */
+  int chunk_used;    /* out: chunk_paths actually used */
+  int order_used;    /* out */
+  int graph_replayed; /* out: 1 if an existing graph was replayed */
+  int64_t work_items; /* out */
+} mcr_batch_opts;
+
+/* Runs nq independent queries.  Persistent mode: every query uses the bf16x3
+ * MFMA kernel family (precision MCR_PREC_BF16X3 or MCR_PREC_AUTO; no per-path
+ * outputs); results equal those of mcr_run with kernel MCR_KERNEL_MFMA_LDS on
+ * the same query.  rs[i] receives query i's result. */
+int mcr_run_batch(const mcr_query *qs, int nq, mcr_result *rs, mcr_batch_opts *opts);
+
+/* Times candidate (chunk_paths, order) settings on this batch, keeps the
+ * fastest as the library's tuned default (used when chunk_paths = 0) and
+ * returns it in *best (chunk_used, order_used, ms_device).  n_candidates
+ * (optional) receives how many settings were timed. */
+int mcr_autotune_batch(const mcr_query *qs, int nq, mcr_batch_opts *best, int *n_candidates);
+
 /* Synthetic factor-model data (host): r_t = drift + B f_t + e_t, t < T, with
  * B ~ U(0.2, 0.8) [n x K], f_t ~ N(0, fvol^2 I_K), e_t ~ N(0, ivol^2 I_n);
  * mu = sample mean, sigma = ddof-1 sample covariance [n*n].  returns (optional)

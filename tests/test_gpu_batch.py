@@ -1,0 +1,112 @@
+"""Batched uneven portfolios: batched factor/prepare + one persistent path
+launch (optionally a replayed hipGraph) against the same queries run alone.
+
+Bar: every per-query result (count, VaR, tail sum, ES) is identical to the
+query run alone on the same kernel family (MCR_KERNEL_MFMA_LDS): the paths are
+keyed by (seed, path id) and all reductions are integer or exact order
+statistics, so batching cannot change them."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from helpers import factor_like
+
+pytestmark = pytest.mark.gpu
+
+
+def portfolios(seed=0, sizes=(5, 17, 50, 64, 100, 128, 200, 333, 600), paths=None):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i, n in enumerate(sizes):
+        mu, sigma = factor_like(n, max(1, min(20, n // 10)), 500 + 7 * i + seed,
+                                T=max(1500, 2 * n))
+        m = int(paths[i]) if paths is not None else int(rng.integers(1000, 60000))
+        out.append({"mu": mu, "sigma": sigma, "x": float(rng.uniform(0.01, 0.04)),
+                    "k": max(1, n // 10), "paths": m, "seed": int(rng.integers(0, 2**63)),
+                    "weights": rng.dirichlet(np.ones(n)), "alphas": (0.99,)})
+    return out
+
+
+def alone(mcr, p, extra_flags=0):
+    return mcr.run(p["mu"], p["sigma"], p["x"], p["k"], p["paths"], p["seed"],
+                   weights=p["weights"], precision=mcr.PREC_BF16X3,
+                   kernel=mcr.KERNEL_MFMA_LDS, alphas=p["alphas"], extra_flags=extra_flags)
+
+
+def same(a, b):
+    assert a.count == b.count
+    assert a.var == b.var
+    assert a.tail_sum_fixed == b.tail_sum_fixed
+    assert a.es == b.es
+
+
+def test_batch_equals_alone(mcr):
+    ps = portfolios(1, paths=[1, 63, 64, 65, 20000, 12345, 30000, 5000, 4000])
+    res, o = mcr.run_batch(ps, use_graph=False, chunk_paths=1024)
+    assert o.work_items >= len(ps)
+    for p, r in zip(ps, res):
+        same(r, alone(mcr, p))
+        assert r.kernel == "mfma_lds"
+
+
+def test_graph_replay_uses_new_inputs(mcr):
+    a = portfolios(2)
+    b = portfolios(3, paths=[p["paths"] for p in a])  # same shapes, new data and seeds
+    r1, o1 = mcr.run_batch(a, use_graph=True, chunk_paths=2048)
+    assert o1.graph_replayed == 0
+    # eager work between replays
+    eager = alone(mcr, a[2])
+    same(eager, r1[2])
+    r2, o2 = mcr.run_batch(b, use_graph=True, chunk_paths=2048)
+    assert o2.graph_replayed == 1
+    r3, o3 = mcr.run_batch(a, use_graph=True, chunk_paths=2048)
+    assert o3.graph_replayed == 1
+    ref_b, _ = mcr.run_batch(b, use_graph=False, chunk_paths=2048)
+    for x, y in zip(r2, ref_b):
+        same(x, y)
+    for x, y in zip(r3, r1):
+        same(x, y)
+
+
+@pytest.mark.parametrize("chunk,order", [(64, 0), (512, 1), (65536, 0)])
+def test_chunking_and_order_do_not_change_results(mcr, chunk, order):
+    ps = portfolios(4)
+    ref, _ = mcr.run_batch(ps, use_graph=False, chunk_paths=4096, order=0)
+    res, _ = mcr.run_batch(ps, use_graph=False, chunk_paths=chunk, order=order)
+    for x, y in zip(res, ref):
+        same(x, y)
+
+
+def test_batch_refinement_fallback(mcr):
+    ps = portfolios(5, sizes=(20, 40))
+    ps[1]["alphas"] = (0.95, 0.99)
+    ps[1]["extra_flags"] = mcr.F_TEST_NARROW  # zero-width windows: the bracket must miss
+    res, _ = mcr.run_batch(ps, use_graph=False, chunk_paths=1024)
+    assert res[1].var_passes > 1 and res[0].var_passes == 1
+    same(res[1], alone(mcr, ps[1]))
+    same(res[0], alone(mcr, ps[0]))
+
+
+def test_per_query_mode_agrees_statistically(mcr):
+    ps = portfolios(6, sizes=(30, 90, 250), paths=[200000, 200000, 100000])
+    per, o = mcr.run_batch(ps, mode=mcr.BATCH_PER_QUERY)
+    bat, _ = mcr.run_batch(ps, use_graph=False)
+    for a, b in zip(per, bat):
+        assert abs(a.p_hat - b.p_hat) <= 1e-3 + 4 * a.std_error
+
+
+def test_autotuner_picks_a_setting(mcr):
+    ps = portfolios(7, sizes=(16, 48, 150), paths=[30000, 30000, 20000])
+    best, tried = mcr.autotune_batch(ps)
+    assert tried == 8 and best.chunk_used in (512, 2048, 8192, 32768)
+    _, o = mcr.run_batch(ps, use_graph=False)  # chunk_paths = 0: the tuned default
+    assert o.chunk_used == best.chunk_used and o.order_used == best.order_used
+
+
+def test_batch_rejects_fp32_queries(mcr):
+    ps = portfolios(8, sizes=(10,))
+    ps[0]["precision"] = mcr.PREC_FP32
+    with pytest.raises(mcr.McrError) as e:
+        mcr.run_batch(ps)
+    assert e.value.code == mcr.MCR_E_UNSUPPORTED
