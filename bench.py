@@ -23,6 +23,8 @@ PEAK_BF16_TFLOPS = 2500.0   # MI355X published dense bf16 MFMA
 
 WORKLOAD = {"name": "fused_n50_100M", "n": 50, "factors": 5, "days": 2000, "paths": 100_000_000,
             "k": 10, "x": 0.02, "alpha": 0.99, "data_seed": 1002, "path_seed": 20251202}
+BATCH = {"name": "batch_256_uneven", "count": 256, "n_range": (5, 600), "paths_range": (1e5, 1e7),
+         "x_range": (0.01, 0.04), "alpha": 0.99, "seed": 1004}
 LARGE = {"name": "large_n600_10M", "n": 600, "factors": 20, "days": 2500, "paths": 10_000_000,
          "k": 10, "x": 0.02, "alphas": (0.99, 0.999), "data_seed": 1003, "path_seed": 20251203}
 
@@ -36,6 +38,7 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--cpu-seconds", type=float, default=12.0)
     p.add_argument("--no-large", action="store_true", help="skip the n=600 leg")
+    p.add_argument("--no-batch", action="store_true", help="skip the 256-portfolio leg")
     p.add_argument("--copy-mb", type=int, default=0,
                    help="first run a plain device copy of this many MiB (counter calibration)")
     return p.parse_args()
@@ -64,6 +67,52 @@ def cpu_baseline(L, mu, x, k, seed, n, budget_s):
             "sample": f"{npar} paths of the n=50 workload (OpenMP, {threads} threads); "
                       f"serial 1-core rate {serial_rate:.4g} paths/s on {ns} paths",
             "serial_value": serial_rate}
+
+
+def batch_portfolios(mcr, np, count=None):
+    """Config 4: n log-uniform in [5, 600], paths log-uniform in [1e5, 1e7],
+    Dirichlet(1) weights, thresholds U(1%, 4%); Sigma from the seeded factor
+    model with K = clamp(n/30, 1, 20) factors over T = 2n + 200 days."""
+    B = BATCH
+    rng = np.random.default_rng(B["seed"])
+    out = []
+    for i in range(count or B["count"]):
+        n = int(round(np.exp(rng.uniform(np.log(B["n_range"][0]), np.log(B["n_range"][1])))))
+        m = int(round(np.exp(rng.uniform(np.log(B["paths_range"][0]), np.log(B["paths_range"][1])))))
+        K = int(min(20, max(1, n // 30)))
+        mu, sigma = mcr.synth_factor_model(n, K, 2 * n + 200, B["seed"] * 1000 + i)
+        out.append({"mu": mu, "sigma": sigma, "x": float(rng.uniform(*B["x_range"])),
+                    "k": max(1, (n + 9) // 10), "paths": m, "seed": int(rng.integers(0, 2**62)),
+                    "weights": rng.dirichlet(np.ones(n)), "alphas": (B["alpha"],),
+                    "precision": mcr.PREC_BF16X3})
+    return out
+
+
+def batch_leg(mcr, np):
+    """Config 4: one persistent launch (hipGraph) vs one launch per portfolio."""
+    ps = batch_portfolios(mcr, np)
+    total = sum(p["paths"] for p in ps)
+    best, tried = mcr.autotune_batch(ps)
+    r0, o0 = mcr.run_batch(ps, use_graph=True)  # captures
+    times = []
+    for _ in range(3):
+        r1, o1 = mcr.run_batch(ps, use_graph=True)
+        times.append(o1.ms_device)
+    per, op = mcr.run_batch(ps, mode=mcr.BATCH_PER_QUERY)
+    ms_b = float(np.median(times))
+    res = {"workload": BATCH["name"], "portfolios": len(ps), "total_paths": total,
+           "n_min": min(p["n"] if "n" in p else len(p["mu"]) for p in ps),
+           "n_max": max(len(p["mu"]) for p in ps),
+           "autotune": {"candidates": tried, "chunk_paths": best.chunk_used,
+                        "order": "lpt" if best.order_used == 0 else "natural"},
+           "persistent_graph_ms": ms_b, "persistent_paths_per_s": total / (ms_b * 1e-3),
+           "graph_replayed": bool(o1.graph_replayed), "work_items": o1.work_items,
+           "per_portfolio_ms": op.ms_total, "per_portfolio_device_ms": op.ms_device,
+           "per_portfolio_paths_per_s": total / (op.ms_total * 1e-3),
+           "speedup_vs_per_portfolio": op.ms_total / ms_b}
+    print(f"# batch: persistent+graph {ms_b:.1f} ms, per-portfolio {op.ms_total:.1f} ms "
+          f"({total:.3g} paths)", file=sys.stderr, flush=True)
+    return res
 
 
 def large_leg(mcr, np):
@@ -162,6 +211,8 @@ def main():
     out.update(extra)
     if not args.no_large:
         out["large"] = large_leg(mcr, np)
+    if not args.no_batch:
+        out["batch"] = batch_leg(mcr, np)
     if not args.no_cpu:
         L, *_ = mcr.factor(sigma)
         out["cpu_baseline"] = cpu_baseline(L, mu, W["x"], W["k"], W["path_seed"], n,

@@ -22,11 +22,13 @@
 #include <vector>
 
 #include "../../include/mcr_hip.h"
+#include "mcr_host.h"
 #include "mcr_internal.h"
 
 using namespace mcr;
 
-namespace {
+namespace mcr {
+namespace host {
 
 thread_local char g_err[512] = "";
 
@@ -57,19 +59,6 @@ constexpr uint64_t kLaunchPaths = 1ull << 32;   // paths per launch (keeps lane 
 constexpr uint64_t kPerPathChunk = 1ull << 24;  // per-path outputs staged per launch
 
 size_t align_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
-
-struct Context {
-  bool ready = false;
-  int device = 0;
-  int num_cus = 0;
-  hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  void* arena = nullptr;
-  size_t arena_bytes = 0;
-  void* host = nullptr;  // pinned staging
-  size_t host_bytes = 0;
-  uint64_t arena_gen = 0;  // bumped whenever the arena moves (invalidates graphs)
-};
 
 Context g_ctx;
 std::mutex g_mu;
@@ -1083,7 +1072,10 @@ int run_batch_locked(const mcr_query* qs, int nq, mcr_result* rs, mcr_batch_opts
   return MCR_OK;
 }
 
-}  // namespace
+}  // namespace host
+}  // namespace mcr
+
+using namespace mcr::host;
 
 extern "C" {
 
