@@ -25,6 +25,9 @@ WORKLOAD = {"name": "fused_n50_100M", "n": 50, "factors": 5, "days": 2000, "path
             "k": 10, "x": 0.02, "alpha": 0.99, "data_seed": 1002, "path_seed": 20251202}
 BATCH = {"name": "batch_256_uneven", "count": 256, "n_range": (5, 600), "paths_range": (1e5, 1e7),
          "x_range": (0.01, 0.04), "alpha": 0.99, "seed": 1004}
+SERVE = {"name": "serve_poisson_200qps", "qps": 200.0, "seconds": 5.0, "reuse": 0.3,
+         "n_range": (5, 600), "paths_range": (1e5, 1e7), "x_range": (0.01, 0.04), "alpha": 0.99,
+         "seed": 1005}
 LARGE = {"name": "large_n600_10M", "n": 600, "factors": 20, "days": 2500, "paths": 10_000_000,
          "k": 10, "x": 0.02, "alphas": (0.99, 0.999), "data_seed": 1003, "path_seed": 20251203}
 
@@ -39,6 +42,8 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=12.0)
     p.add_argument("--no-large", action="store_true", help="skip the n=600 leg")
     p.add_argument("--no-batch", action="store_true", help="skip the 256-portfolio leg")
+    p.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    p.add_argument("--serve-seconds", type=float, default=SERVE["seconds"])
     p.add_argument("--copy-mb", type=int, default=0,
                    help="first run a plain device copy of this many MiB (counter calibration)")
     return p.parse_args()
@@ -112,6 +117,102 @@ def batch_leg(mcr, np):
            "speedup_vs_per_portfolio": op.ms_total / ms_b}
     print(f"# batch: persistent+graph {ms_b:.1f} ms, per-portfolio {op.ms_total:.1f} ms "
           f"({total:.3g} paths)", file=sys.stderr, flush=True)
+    return res
+
+
+def serve_queries(mcr, np, count, seed):
+    """Config 5 stream: arrival gaps ~ Exp(1/qps); sizes as config 4; a query
+    reuses an earlier query's covariance (and mean) with probability 0.3."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    S = SERVE
+    rng = np.random.default_rng(seed)
+    shape, src = [], []
+    for i in range(count):
+        if i and rng.uniform() < S["reuse"]:
+            src.append(src[int(rng.integers(0, i))])
+        else:
+            src.append(i)
+        n = int(round(np.exp(rng.uniform(np.log(S["n_range"][0]), np.log(S["n_range"][1])))))
+        shape.append(n)
+    distinct = sorted(set(src))
+
+    def make(i):
+        n = shape[i]
+        return mcr.synth_factor_model(n, int(min(20, max(1, n // 30))), 2 * n + 200,
+                                      seed * 1000 + i)
+
+    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        data = dict(zip(distinct, ex.map(make, distinct)))
+    gaps = rng.exponential(1.0 / S["qps"], count)
+    out = []
+    for i in range(count):
+        mu, sigma = data[src[i]]
+        n = mu.shape[0]
+        m = int(round(np.exp(rng.uniform(np.log(S["paths_range"][0]), np.log(S["paths_range"][1])))))
+        out.append({"mu": mu, "sigma": sigma, "x": float(rng.uniform(*S["x_range"])),
+                    "k": max(1, (n + 9) // 10), "paths": m, "seed": int(rng.integers(0, 2**62)),
+                    "weights": rng.dirichlet(np.ones(n)), "alphas": (S["alpha"],),
+                    "precision": mcr.PREC_BF16X3})
+    return out, gaps, len(distinct)
+
+
+def serve_leg(mcr, np, seconds):
+    """Config 5: open-loop Poisson arrivals into the serving loop; latency is
+    submit -> result ready as the server records it."""
+    import threading
+
+    S = SERVE
+    count = max(1, int(S["qps"] * seconds))
+    qs, gaps, n_distinct = serve_queries(mcr, np, count, S["seed"])
+    warm, _, _ = serve_queries(mcr, np, 8, S["seed"] + 1)
+    infos = [None] * count
+    with mcr.Server() as srv:
+        # warm-up on other queries (kernel load, cost-model calibration)
+        for t in [srv.submit(q) for q in warm]:
+            srv.wait(t)
+        tickets = [None] * count
+        ready = threading.Semaphore(0)
+
+        def collect():
+            for i in range(count):
+                ready.acquire()
+                infos[i] = srv.wait(tickets[i])[1]
+
+        th = threading.Thread(target=collect)
+        th.start()
+        st0 = srv.stats()
+        t0 = time.perf_counter()
+        due = t0
+        for i in range(count):
+            due += gaps[i]
+            d = due - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            tickets[i] = srv.submit(qs[i])
+            ready.release()
+        submit_span = time.perf_counter() - t0
+        th.join()
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+    lat = np.array([x.latency_ms for x in infos])
+    que = np.array([x.queue_ms for x in infos])
+    paths = sum(q["paths"] for q in qs)
+    hits = st.cache_hits - st0.cache_hits
+    res = {"workload": S["name"], "queries": count, "offered_qps": S["qps"],
+           "achieved_qps": count / submit_span, "distinct_covariances": n_distinct,
+           "total_paths": paths, "sustained_paths_per_s": paths / wall,
+           "latency_ms": {"p50": float(np.percentile(lat, 50)), "p90": float(np.percentile(lat, 90)),
+                          "p99": float(np.percentile(lat, 99)), "max": float(lat.max()),
+                          "mean": float(lat.mean())},
+           "queue_ms": {"p50": float(np.percentile(que, 50)), "p99": float(np.percentile(que, 99))},
+           "rounds": st.rounds - st0.rounds,
+           "device_busy_frac": (st.device_ms - st0.device_ms) / (wall * 1e3),
+           "cache_hit_rate": hits / count, "peak_device_bytes": st.peak_bytes,
+           "budget_bytes": st.budget_bytes, "failed": st.failed}
+    print(f"# serve: {count} queries at {res['achieved_qps']:.0f} QPS, p50 {res['latency_ms']['p50']:.1f} ms,"
+          f" p99 {res['latency_ms']['p99']:.1f} ms, {res['sustained_paths_per_s']:.3g} paths/s, "
+          f"busy {res['device_busy_frac']:.2f}, cache hits {hits}", file=sys.stderr, flush=True)
     return res
 
 
@@ -213,6 +314,8 @@ def main():
         out["large"] = large_leg(mcr, np)
     if not args.no_batch:
         out["batch"] = batch_leg(mcr, np)
+    if not args.no_serve:
+        out["serving"] = serve_leg(mcr, np, args.serve_seconds)
     if not args.no_cpu:
         L, *_ = mcr.factor(sigma)
         out["cpu_baseline"] = cpu_baseline(L, mu, W["x"], W["k"], W["path_seed"], n,

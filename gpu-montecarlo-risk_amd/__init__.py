@@ -310,6 +310,108 @@ def autotune_batch(portfolios):
     return o, nc.value
 
 
+class ServerOpts(ctypes.Structure):
+    _fields_ = [("hbm_budget", ctypes.c_int64), ("cache_entries", ctypes.c_int),
+                ("max_active", ctypes.c_int), ("round_ms", ctypes.c_double),
+                ("grid", ctypes.c_int)]
+
+
+class TicketInfo(ctypes.Structure):
+    _fields_ = [("latency_ms", ctypes.c_double), ("queue_ms", ctypes.c_double),
+                ("rounds", ctypes.c_int), ("cache_hit", ctypes.c_int), ("status", ctypes.c_int)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_int64), ("completed", ctypes.c_int64),
+                ("failed", ctypes.c_int64), ("rounds", ctypes.c_int64),
+                ("cache_hits", ctypes.c_int64), ("cache_misses", ctypes.c_int64),
+                ("paths_done", ctypes.c_int64), ("device_ms", ctypes.c_double),
+                ("peak_bytes", ctypes.c_int64), ("budget_bytes", ctypes.c_int64),
+                ("active", ctypes.c_int), ("queued", ctypes.c_int)]
+
+
+def _server_lib():
+    L = lib()
+    if not getattr(L, "_server_sigs", False):
+        vp = ctypes.c_void_p
+        L.mcr_server_footprint.argtypes = [ctypes.POINTER(Query), ctypes.POINTER(ctypes.c_int64)]
+        L.mcr_server_create.argtypes = [ctypes.c_int, ctypes.POINTER(ServerOpts),
+                                        ctypes.POINTER(vp)]
+        L.mcr_server_submit.argtypes = [vp, ctypes.POINTER(Query), ctypes.POINTER(ctypes.c_uint64)]
+        L.mcr_server_wait.argtypes = [vp, ctypes.c_uint64, ctypes.POINTER(Result),
+                                      ctypes.POINTER(TicketInfo)]
+        L.mcr_server_poll.argtypes = [vp, ctypes.c_uint64, ctypes.POINTER(Result),
+                                      ctypes.POINTER(TicketInfo)]
+        L.mcr_server_get_stats.argtypes = [vp, ctypes.POINTER(ServerStats)]
+        L.mcr_server_destroy.argtypes = [vp]
+        L._server_sigs = True
+    return L
+
+
+def server_footprint(portfolio) -> int:
+    """Device bytes a query holds while admitted (mcr_server_footprint; host only)."""
+    qs, keep = _queries([portfolio])
+    b = ctypes.c_int64(0)
+    _check(_server_lib().mcr_server_footprint(ctypes.byref(qs[0]), ctypes.byref(b)))
+    del keep
+    return b.value
+
+
+class Server:
+    """Serving loop (mcr_server_*): submit queries as they arrive, wait for
+    their results.  Portfolios are dicts as for run_batch."""
+
+    def __init__(self, device: int = 0, *, hbm_budget: int = 0, cache_entries: int = 0,
+                 max_active: int = 0, round_ms: float = 0.0, grid: int = 0):
+        self._L = _server_lib()
+        self._h = ctypes.c_void_p()
+        o = ServerOpts(hbm_budget, cache_entries, max_active, round_ms, grid)
+        _check(self._L.mcr_server_create(device, ctypes.byref(o), ctypes.byref(self._h)))
+        self._na = {}
+
+    def submit(self, portfolio) -> int:
+        qs, keep = _queries([portfolio])
+        t = ctypes.c_uint64(0)
+        _check(self._L.mcr_server_submit(self._h, ctypes.byref(qs[0]), ctypes.byref(t)))
+        del keep  # the server copied the inputs
+        self._na[t.value] = len(portfolio.get("alphas", ()))
+        return t.value
+
+    def wait(self, ticket: int):
+        """(RiskResult, TicketInfo) of a ticket; raises McrError on failure."""
+        r, info = Result(), TicketInfo()
+        code = self._L.mcr_server_wait(self._h, ticket, ctypes.byref(r), ctypes.byref(info))
+        na = self._na.pop(ticket, 0)
+        _check(code)
+        return _result(r, na), info
+
+    def poll(self, ticket: int):
+        """None while pending, else as wait()."""
+        r, info = Result(), TicketInfo()
+        code = self._L.mcr_server_poll(self._h, ticket, ctypes.byref(r), ctypes.byref(info))
+        if code == 0:
+            return None
+        na = self._na.pop(ticket, 0)
+        _check(0 if code == 1 else code)
+        return _result(r, na), info
+
+    def stats(self) -> ServerStats:
+        s = ServerStats()
+        _check(self._L.mcr_server_get_stats(self._h, ctypes.byref(s)))
+        return s
+
+    def close(self) -> None:
+        if self._h:
+            _check(self._L.mcr_server_destroy(self._h))
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
 def factor(sigma):
     """GPU FP64 Cholesky with the jitter policy: (L, attempts, jitter, clamped)."""
     sigma = np.ascontiguousarray(sigma, dtype=np.float64)

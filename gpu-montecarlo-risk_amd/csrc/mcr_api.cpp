@@ -39,26 +39,8 @@ void set_error(const char* fmt, ...) {
   va_end(ap);
 }
 
-#define HIP_TRY(expr)                                                              \
-  do {                                                                             \
-    hipError_t e_ = (expr);                                                        \
-    if (e_ != hipSuccess) {                                                        \
-      set_error("%s failed: %s (line %d)", #expr, hipGetErrorString(e_), __LINE__); \
-      return MCR_E_DEVICE;                                                         \
-    }                                                                              \
-  } while (0)
-
-#define MCR_TRY(expr)     \
-  do {                    \
-    int r_ = (expr);      \
-    if (r_ != MCR_OK) return r_; \
-  } while (0)
-
-constexpr size_t kAlign = 256;
 constexpr uint64_t kLaunchPaths = 1ull << 32;   // paths per launch (keeps lane tallies 32-bit)
 constexpr uint64_t kPerPathChunk = 1ull << 24;  // per-path outputs staged per launch
-
-size_t align_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
 
 Context g_ctx;
 std::mutex g_mu;
@@ -125,16 +107,6 @@ int reserve_host(size_t bytes) {
 }
 
 // Carves 256-byte aligned pieces out of a base pointer (device or host).
-struct Carver {
-  char* base;
-  size_t off = 0;
-  template <typename T>
-  T* take(size_t count) {
-    T* p = base ? reinterpret_cast<T*>(base + off) : nullptr;
-    off += align_up(count * sizeof(T));
-    return p;
-  }
-};
 
 // Inverse standard normal cdf by Newton on erfc from a bisection bracket.
 double inv_norm_cdf(double p) {
@@ -349,17 +321,7 @@ int stage_query(const mcr_query* q, Staged* st, bool want_var_windows) {
   fa.L = st->d.L;
   fa.v64 = st->d.v64;
   fa.n = q->n;
-  fa.n_alpha = want_var_windows ? p.n_alpha : 0;
-  for (int i = 0; i < fa.n_alpha; ++i) {
-    const double a = q->alpha[i];
-    const double za = inv_norm_cdf(a);
-    const double dens = std::exp(-0.5 * za * za) / std::sqrt(2.0 * M_PI);
-    // six standard errors of the empirical quantile plus a margin for the
-    // fp32 stream's departure from the exact Gaussian
-    fa.za[i] = za;
-    fa.ha[i] = (q->flags & MCR_F_TEST_NARROW) ? 0.0
-                                  : 6.0 * std::sqrt(a * (1.0 - a) / (double)q->paths) / dens + 1e-4;
-  }
+  set_windows(q, want_var_windows ? p.n_alpha : 0, &fa);
   fa.info = st->d.info;
   fa.acc = st->d.acc;
   fa.hist = st->d.hist;
@@ -627,18 +589,8 @@ int run_locked(const mcr_query* q, mcr_result* r) {
   MCR_TRY(check_factor(info));
 
   const int64_t M = q->paths;
-  r->count = (int64_t)acc.count;
+  fill_statistics(M, info, acc, r);
   r->near_threshold = (int64_t)acc.near;
-  r->p_hat = (double)r->count / (double)M;
-  r->std_error = std::sqrt(r->p_hat * (1.0 - r->p_hat) / (double)M);
-  const double margin = 1.96 * r->std_error;
-  r->ci_lower = std::fmax(0.0, r->p_hat - margin);
-  r->ci_upper = std::fmin(1.0, r->p_hat + margin);
-  r->loss_mean = -info.mp;
-  r->loss_sd = info.loss_sd;
-  r->jitter = info.jitter;
-  r->factor_attempts = info.attempts;
-  r->factor_clamped = info.clamped;
   r->kernel = st.plan.kernel;
   r->ms_kernel = ms;
   r->var_passes = (q->flags & MCR_F_VAR) ? 1 : 0;
@@ -668,6 +620,37 @@ int run_locked(const mcr_query* q, mcr_result* r) {
   return MCR_OK;
 }
 
+
+// VaR windows of a query: centre Phi^-1(alpha), half-width six standard errors
+// of the empirical quantile (in loss sd units) plus 1e-4 sd for the fp32
+// stream's departure from the exact Gaussian.
+void set_windows(const mcr_query* q, int n_alpha, FactorArgs* f) {
+  f->n_alpha = n_alpha;
+  for (int al = 0; al < n_alpha; ++al) {
+    const double av = q->alpha[al];
+    const double za = inv_norm_cdf(av);
+    const double dens = std::exp(-0.5 * za * za) / std::sqrt(2.0 * M_PI);
+    f->za[al] = za;
+    f->ha[al] = (q->flags & MCR_F_TEST_NARROW)
+                    ? 0.0
+                    : 6.0 * std::sqrt(av * (1.0 - av) / (double)q->paths) / dens + 1e-4;
+  }
+}
+
+// P-hat, its standard error and 95% interval (src/05-GPU/monte_carlos_cuda.cu:146-150),
+// plus the factor outcome and loss moments.
+void fill_statistics(int64_t M, const DevInfo& info, const DevAccum& acc, mcr_result* r) {
+  r->count = (int64_t)acc.count;
+  r->p_hat = (double)r->count / (double)M;
+  r->std_error = std::sqrt(r->p_hat * (1.0 - r->p_hat) / (double)M);
+  r->ci_lower = std::fmax(0.0, r->p_hat - 1.96 * r->std_error);
+  r->ci_upper = std::fmin(1.0, r->p_hat + 1.96 * r->std_error);
+  r->loss_mean = -info.mp;
+  r->loss_sd = info.loss_sd;
+  r->jitter = info.jitter;
+  r->factor_attempts = info.attempts;
+  r->factor_clamped = info.clamped;
+}
 
 // ---------------------------------------------------------------------------
 // Batches: batched factor (one workgroup per query), batched prepare, then one
@@ -851,16 +834,7 @@ void fill_batch_inputs(const mcr_query* qs, const BatchPlan& bp, char* h, char* 
     f.L = (double*)(dev + bp.offL[i]);
     f.v64 = (double*)(dev + bp.offV[i]);
     f.n = q->n;
-    f.n_alpha = bp.na[i];
-    for (int al = 0; al < f.n_alpha; ++al) {
-      const double av = q->alpha[al];
-      const double za = inv_norm_cdf(av);
-      const double dens = std::exp(-0.5 * za * za) / std::sqrt(2.0 * M_PI);
-      f.za[al] = za;
-      f.ha[al] = (q->flags & MCR_F_TEST_NARROW)
-                     ? 0.0
-                     : 6.0 * std::sqrt(av * (1.0 - av) / (double)q->paths) / dens + 1e-4;
-    }
+    set_windows(q, bp.na[i], &f);
     f.info = info;
     f.acc = acc;
     f.hist = (unsigned*)(dev + bp.offHist);
@@ -1024,16 +998,7 @@ int run_batch_locked(const mcr_query* qs, int nq, mcr_result* rs, mcr_batch_opts
       return fr;
     }
     const int64_t M = q->paths;
-    r->count = (int64_t)accs[i].count;
-    r->p_hat = (double)r->count / (double)M;
-    r->std_error = std::sqrt(r->p_hat * (1.0 - r->p_hat) / (double)M);
-    r->ci_lower = std::fmax(0.0, r->p_hat - 1.96 * r->std_error);
-    r->ci_upper = std::fmin(1.0, r->p_hat + 1.96 * r->std_error);
-    r->loss_mean = -infos[i].mp;
-    r->loss_sd = infos[i].loss_sd;
-    r->jitter = infos[i].jitter;
-    r->factor_attempts = infos[i].attempts;
-    r->factor_clamped = infos[i].clamped;
+    fill_statistics(M, infos[i], accs[i], r);
     r->kernel = MCR_KERNEL_MFMA_LDS;
     r->ms_kernel = ms;
     r->var_passes = bp.na[i] ? 1 : 0;

@@ -43,18 +43,15 @@ __device__ double block_sum_f64(double v, double* red) {
   return s;
 }
 
-__device__ void factor_body(const FactorArgs& a, double* col, double* red) {
+// FP64 Crout factorisation of A into L with diagonal jitter: attempt t adds
+// delta_t = mean(diag A) * 10^(min(t,8)-12) (t >= 1); attempt 9 clamps failing
+// pivots to delta.  Returns false (and records attempts = -1) for non-finite A.
+__device__ bool cholesky_body(const FactorArgs& a, double* col, double* red, int* attempts_out,
+                              int* clamped_out, int* fail_out, double* delta_out) {
   const int n = a.n;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const double* __restrict__ A = a.A;
   double* __restrict__ L = a.L;
-
-  // Clear the query's accumulators.
-  if (tid == 0) {
-    DevAccum z = {};
-    *a.acc = z;
-  }
-  for (int i = tid; i < kHistBins; i += blockDim.x) a.hist[i] = 0u;
 
   double bad = 0.0, dsum = 0.0;
   for (int e = tid; e < n * n; e += blockDim.x) {
@@ -71,8 +68,9 @@ __device__ void factor_body(const FactorArgs& a, double* col, double* red) {
       a.info->clamped = 0;
       a.info->fail_col = 0;
       a.info->jitter = 0.0;
+      if (a.finfo) *a.finfo = *a.info;
     }
-    return;
+    return false;
   }
   const double mean = dsum / (double)n;
 
@@ -115,8 +113,21 @@ __device__ void factor_body(const FactorArgs& a, double* col, double* red) {
     last_fail = fail;
     if (!fail) break;
   }
+  *attempts_out = attempts;
+  *clamped_out = clamped;
+  *fail_out = last_fail;
+  *delta_out = delta;
+  return true;
+}
 
-  // Portfolio loading v = L^T w, mean w^T mu and sd ||v|| in FP64.
+// Portfolio loading v = L^T w, mean w^T mu and sd ||v|| in FP64, and the
+// query's record (factor outcome, moments, VaR windows).
+__device__ void moments_body(const FactorArgs& a, double* red, int attempts, int clamped,
+                             int last_fail, double delta) {
+  const int n = a.n;
+  const int tid = threadIdx.x;
+  const double* __restrict__ L = a.L;
+  __syncthreads();
   for (int j = tid; j < n; j += blockDim.x) {
     double s = 0.0;
     for (int i = j; i < n; ++i) s += L[(size_t)i * n + j] * a.w[i];
@@ -151,6 +162,47 @@ __device__ void factor_body(const FactorArgs& a, double* col, double* red) {
     info->hist_lo = 0.0f;
     info->hist_scale = 0.0f;
   }
+}
+
+
+__device__ void factor_body(const FactorArgs& a, double* col, double* red) {
+  const int tid = threadIdx.x;
+
+  // Clear the query's accumulators.
+  if (tid == 0) {
+    DevAccum z = {};
+    *a.acc = z;
+  }
+  for (int i = tid; i < kHistBins; i += blockDim.x) a.hist[i] = 0u;
+
+  int attempts = 0, clamped = 0, last_fail = 0;
+  double delta = 0.0;
+  if (a.reuse) {
+    // L holds this covariance's factor from an earlier launch.
+    const DevInfo fi = *a.finfo;
+    attempts = fi.attempts;
+    clamped = fi.clamped;
+    last_fail = fi.fail_col;
+    delta = fi.jitter;
+    if (attempts < 0) {
+      if (tid == 0) {
+        a.info->attempts = -1;
+        a.info->clamped = 0;
+        a.info->fail_col = 0;
+        a.info->jitter = 0.0;
+      }
+      return;
+    }
+  } else if (!cholesky_body(a, col, red, &attempts, &clamped, &last_fail, &delta)) {
+    return;
+  }
+  if (a.finfo && !a.reuse && tid == 0) {
+    a.finfo->attempts = attempts;
+    a.finfo->clamped = clamped;
+    a.finfo->fail_col = last_fail;
+    a.finfo->jitter = delta;
+  }
+  moments_body(a, red, attempts, clamped, last_fail, delta);
 }
 
 __global__ __launch_bounds__(1024) void k_factor(FactorArgs a) {

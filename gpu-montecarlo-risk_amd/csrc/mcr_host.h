@@ -9,8 +9,39 @@
 #include "../../include/mcr_hip.h"
 #include "mcr_internal.h"
 
+#define HIP_TRY(expr)                                                              \
+  do {                                                                             \
+    hipError_t e_ = (expr);                                                        \
+    if (e_ != hipSuccess) {                                                        \
+      ::mcr::host::set_error("%s failed: %s (line %d)", #expr, hipGetErrorString(e_), \
+                             __LINE__);                                            \
+      return MCR_E_DEVICE;                                                         \
+    }                                                                              \
+  } while (0)
+
+#define MCR_TRY(expr)              \
+  do {                             \
+    int r_ = (expr);               \
+    if (r_ != MCR_OK) return r_;   \
+  } while (0)
+
 namespace mcr {
 namespace host {
+
+constexpr size_t kAlign = 256;
+inline size_t align_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
+
+// Bump allocator over a region: base == nullptr only measures.
+struct Carver {
+  char* base;
+  size_t off = 0;
+  template <typename T>
+  T* take(size_t count) {
+    T* p = base ? reinterpret_cast<T*>(base + off) : nullptr;
+    off += align_up(count * sizeof(T));
+    return p;
+  }
+};
 
 struct Context {
   bool ready = false;
@@ -39,7 +70,8 @@ double group_cost(int T, int S);
 int resolve_quantile(int64_t M, const DevAccum& acc, int slot, const float* gather, unsigned cap,
                      int64_t R, double* var, double* es, int64_t* tail);
 int run_locked(const mcr_query* q, mcr_result* r);
-size_t align_up(size_t x);
+void set_windows(const mcr_query* q, int n_alpha, FactorArgs* f);
+void fill_statistics(int64_t M, const DevInfo& info, const DevAccum& acc, mcr_result* r);
 
 }  // namespace host
 }  // namespace mcr

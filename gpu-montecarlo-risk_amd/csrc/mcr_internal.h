@@ -87,6 +87,11 @@ struct FactorArgs {
   DevInfo* info;
   DevAccum* acc;
   unsigned* hist;        // cleared too
+  // Factor record of the covariance (attempts, clamped, fail_col, jitter):
+  // written after factoring when reuse == 0, read when reuse == 1, in which
+  // case L already holds the factor of this covariance and A is not read.
+  DevInfo* finfo;
+  int reuse;
 };
 
 struct PrepArgs {

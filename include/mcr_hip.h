@@ -168,6 +168,63 @@ int mcr_run_batch(const mcr_query *qs, int nq, mcr_result *rs, mcr_batch_opts *o
  * (optional) receives how many settings were timed. */
 int mcr_autotune_batch(const mcr_query *qs, int nq, mcr_batch_opts *best, int *n_candidates);
 
+/* ---- serving loop (continuous batching) ----
+ * No counterpart in the reference (it runs one query per process,
+ * src/main_runner.c:198-296).  A server owns a device-memory arena of
+ * hbm_budget bytes, a cache of FP64 Cholesky factors keyed by covariance
+ * content and a worker thread.  Submitted queries are admitted while their
+ * device footprint fits the free budget; the worker runs rounds of the
+ * persistent path kernel over every admitted query, admitting new queries
+ * between rounds, so queries that arrive while others run share the device
+ * with them.  Each query's paths are keyed by (seed, path id) and reduced
+ * exactly, so its result equals mcr_run with kernel MCR_KERNEL_MFMA_LDS on the
+ * same query.  Queries: precision MCR_PREC_BF16X3 or MCR_PREC_AUTO, flags
+ * MCR_F_VAR only, no per-path outputs.  All calls are thread-safe. */
+typedef struct mcr_server mcr_server;
+
+typedef struct mcr_server_opts {
+  int64_t hbm_budget;  /* device bytes for admitted queries and cached factors; 0 = 4 GiB */
+  int cache_entries;   /* Cholesky factors kept on the device; 0 = 256, -1 = no cache */
+  int max_active;      /* queries admitted at once; 0 = 1024 */
+  double round_ms;     /* target device time of one round; 0 = 4 ms */
+  int grid;            /* persistent workgroups; 0 = one per CU */
+} mcr_server_opts;
+
+typedef struct mcr_ticket_info {
+  double latency_ms;  /* submit -> result ready */
+  double queue_ms;    /* submit -> admitted */
+  int rounds;         /* rounds the query's paths were spread over */
+  int cache_hit;      /* 1 if its Cholesky factor came from the device cache */
+  int status;         /* MCR_OK or the query's error code */
+} mcr_ticket_info;
+
+typedef struct mcr_server_stats {
+  int64_t submitted, completed, failed;
+  int64_t rounds, cache_hits, cache_misses;
+  int64_t paths_done;
+  double device_ms;      /* summed device time of all rounds */
+  int64_t peak_bytes;    /* high-water mark of the budget in use */
+  int64_t budget_bytes;
+  int active, queued;    /* at the time of the call */
+} mcr_server_stats;
+
+/* Device bytes a query needs while admitted (inputs, operands, accumulators,
+ * VaR buffers, plus its factor when not cached); host-only. */
+int mcr_server_footprint(const mcr_query *q, int64_t *bytes);
+
+int mcr_server_create(int device, const mcr_server_opts *opts, mcr_server **out);
+/* Copies the query's inputs and queues it; *ticket identifies it.  Fails with
+ * MCR_E_NOMEM when the query could never fit the budget. */
+int mcr_server_submit(mcr_server *s, const mcr_query *q, uint64_t *ticket);
+/* Blocks until the ticket's query is done; returns its status.  info may be
+ * NULL.  A ticket is consumed by the wait (or poll) that returns it. */
+int mcr_server_wait(mcr_server *s, uint64_t ticket, mcr_result *r, mcr_ticket_info *info);
+/* 1 (and the result) if done, 0 if not yet, < 0 for an unknown ticket. */
+int mcr_server_poll(mcr_server *s, uint64_t ticket, mcr_result *r, mcr_ticket_info *info);
+int mcr_server_get_stats(mcr_server *s, mcr_server_stats *st);
+/* Finishes the admitted queries, fails the queued ones, frees everything. */
+int mcr_server_destroy(mcr_server *s);
+
 /* Synthetic factor-model data (host): r_t = drift + B f_t + e_t, t < T, with
  * B ~ U(0.2, 0.8) [n x K], f_t ~ N(0, fvol^2 I_K), e_t ~ N(0, ivol^2 I_n);
  * mu = sample mean, sigma = ddof-1 sample covariance [n*n].  returns (optional)

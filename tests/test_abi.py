@@ -63,3 +63,28 @@ def test_synth_large_condition_number(mcr):
 def test_bad_arguments_rejected_without_gpu(mcr):
     with pytest.raises(mcr.McrError):
         mcr.synth_factor_model(0, 2, 100, 1)
+
+
+def test_server_footprint_and_servable_checks(mcr):
+    """Host-only admission arithmetic of the serving loop."""
+    n = 600
+    p = {"mu": np.zeros(n), "sigma": np.eye(n), "x": 0.02, "k": 5, "paths": 10**7, "seed": 1,
+         "alphas": (0.99,), "precision": mcr.PREC_BF16X3}
+    b = mcr.server_footprint(p)
+    # factor (FP64 L) + covariance copy + bf16 hi/lo operands + one VaR buffer dominate
+    T, S = (n + 1 + 31) // 32, (n + 15) // 16
+    assert b >= 2 * n * n * 8 + T * S * 128 * 16 + 65536 * 4
+    assert b <= 1.1 * (2 * n * n * 8 + T * S * 128 * 16 + 65536 * 4) + 8192
+    small = dict(p, mu=np.zeros(5), sigma=np.eye(5), alphas=())
+    assert mcr.server_footprint(small) < 16384
+    for bad, code in (({"precision": mcr.PREC_FP32}, mcr.MCR_E_UNSUPPORTED),
+                      ({"extra_flags": mcr.F_NEAR}, mcr.MCR_E_UNSUPPORTED),
+                      ({"paths": 0}, mcr.MCR_E_ARG)):
+        with pytest.raises(mcr.McrError) as e:
+            mcr.server_footprint(dict(p, **bad))
+        assert e.value.code == code
+    mu = np.zeros(n)
+    mu[7] = np.inf
+    with pytest.raises(mcr.McrError) as e:
+        mcr.server_footprint(dict(p, mu=mu))
+    assert e.value.code == mcr.MCR_E_ARG

@@ -1,0 +1,177 @@
+"""Serving loop (mcr_server_*): queries arrive over time, are admitted by
+device-memory footprint, share rounds of the persistent kernel with whatever
+else is running, and reuse cached Cholesky factors.
+
+Bar: every served result (count, VaR, tail sums, ES) is identical to the same
+query run alone on the same kernel family (mcr_run, MCR_KERNEL_MFMA_LDS):
+paths are keyed by (seed, path id) and reduced into integer / fixed-point
+tallies, so the mix, the round boundaries and the cache cannot change them."""
+from __future__ import annotations
+
+import threading
+import time
+
+import numpy as np
+import pytest
+
+from helpers import factor_like
+
+pytestmark = pytest.mark.gpu
+
+
+def stream(seed, count, sizes=(5, 12, 33, 50, 64, 100, 200, 333, 600), reuse=0.3,
+           paths=(2_000, 80_000)):
+    """Queries of mixed sizes; a fraction reuses an earlier covariance."""
+    rng = np.random.default_rng(seed)
+    seen = []
+    out = []
+    for i in range(count):
+        if seen and rng.uniform() < reuse:
+            mu, sigma = seen[int(rng.integers(0, len(seen)))]
+        else:
+            n = int(sizes[int(rng.integers(0, len(sizes)))])
+            mu, sigma = factor_like(n, max(1, min(20, n // 10)), 900 + 13 * i + seed,
+                                    T=max(1500, 2 * n))
+            seen.append((mu, sigma))
+        n = mu.shape[0]
+        out.append({"mu": mu, "sigma": sigma, "x": float(rng.uniform(0.01, 0.04)),
+                    "k": max(1, n // 10), "seed": int(rng.integers(0, 2**63)),
+                    "paths": int(np.exp(rng.uniform(np.log(paths[0]), np.log(paths[1])))),
+                    "weights": rng.dirichlet(np.ones(n)), "alphas": (0.99,)})
+    return out
+
+
+def alone(mcr, p):
+    return mcr.run(p["mu"], p["sigma"], p["x"], p["k"], p["paths"], p["seed"],
+                   weights=p["weights"], precision=mcr.PREC_BF16X3,
+                   kernel=mcr.KERNEL_MFMA_LDS, alphas=p.get("alphas", ()),
+                   extra_flags=p.get("extra_flags", 0))
+
+
+def same(a, b):
+    assert a.count == b.count
+    assert a.var == b.var
+    assert a.tail_sum_fixed == b.tail_sum_fixed
+    assert a.es == b.es
+    assert a.factor_attempts == b.factor_attempts and a.jitter == b.jitter
+
+
+def serve(mcr, ps, gaps_ms=2.0, seed=0, **opts):
+    """Submits ps with exponential gaps (mean gaps_ms), returns results in order."""
+    rng = np.random.default_rng(seed)
+    with mcr.Server(**opts) as srv:
+        tickets = []
+        for p in ps:
+            tickets.append(srv.submit(p))
+            time.sleep(float(rng.exponential(gaps_ms)) / 1e3)
+        out = [srv.wait(t) for t in tickets]
+        st = srv.stats()
+    return out, st
+
+
+def test_served_results_equal_alone(mcr):
+    ps = stream(1, 30)
+    out, st = serve(mcr, ps, round_ms=0.1)
+    assert st.completed == len(ps) and st.failed == 0
+    assert st.cache_hits + st.cache_misses == len(ps)
+    assert st.cache_hits >= 1
+    assert max(info.rounds for _, info in out) > 1  # some query spanned several rounds
+    for p, (r, info) in zip(ps, out):
+        assert info.status == 0 and info.latency_ms >= info.queue_ms >= 0.0
+        assert r.kernel == "mfma_lds"
+        same(r, alone(mcr, p))
+
+
+def test_cache_does_not_change_results(mcr):
+    ps = stream(2, 16, reuse=0.6)
+    cached, st1 = serve(mcr, ps, gaps_ms=0.5)
+    uncached, st2 = serve(mcr, ps, gaps_ms=0.5, cache_entries=-1)
+    assert st1.cache_hits > 0
+    assert st2.cache_hits <= st1.cache_hits
+    for (a, _), (b, _) in zip(cached, uncached):
+        same(a, b)
+
+
+def test_admission_budget_holds(mcr):
+    ps = stream(3, 20, sizes=(100, 333, 600), reuse=0.0, paths=(5_000, 40_000))
+    need = max(mcr.server_footprint(p) for p in ps)
+    budget = int(need * 2.5)
+    out, st = serve(mcr, ps, gaps_ms=0.2, hbm_budget=budget)
+    assert st.completed == len(ps)
+    assert 0 < st.peak_bytes <= budget
+    assert max(info.queue_ms for _, info in out) > 0.0  # someone waited for memory
+    for p, (r, _) in zip(ps, out):
+        same(r, alone(mcr, p))
+
+
+def test_query_larger_than_budget_is_refused(mcr):
+    p = stream(4, 1, sizes=(600,))[0]
+    need = mcr.server_footprint(p)
+    with mcr.Server(hbm_budget=need // 2) as srv:
+        with pytest.raises(mcr.McrError) as e:
+            srv.submit(p)
+        assert e.value.code == mcr.MCR_E_NOMEM
+        small = stream(5, 1, sizes=(5,))[0]
+        r, _ = srv.wait(srv.submit(small))
+        same(r, alone(mcr, small))
+
+
+def test_factor_failure_is_reported_and_not_cached(mcr):
+    ps = stream(6, 3, sizes=(20,))
+    bad = dict(ps[1])
+    bad["sigma"] = bad["sigma"].copy()
+    bad["sigma"][3, 4] = np.nan
+    with mcr.Server() as srv:
+        t = [srv.submit(ps[0]), srv.submit(bad), srv.submit(ps[2])]
+        time.sleep(0.05)
+        t.append(srv.submit(bad))  # same bad covariance again
+        same(srv.wait(t[0])[0], alone(mcr, ps[0]))
+        for tb in (t[1], t[3]):
+            with pytest.raises(mcr.McrError) as e:
+                srv.wait(tb)
+            assert e.value.code == mcr.MCR_E_FACTOR
+        same(srv.wait(t[2])[0], alone(mcr, ps[2]))
+        assert srv.stats().failed == 2
+
+
+def test_quantile_refinement_fallback(mcr):
+    ps = stream(7, 2, sizes=(40,))
+    ps[1]["alphas"] = (0.95, 0.99)
+    ps[1]["extra_flags"] = mcr.F_TEST_NARROW  # zero-width windows force the fallback
+    out, _ = serve(mcr, ps)
+    assert out[1][0].var_passes > 1 and out[0][0].var_passes == 1
+    for p, (r, _) in zip(ps, out):
+        same(r, alone(mcr, p))
+
+
+def test_concurrent_submitters_and_poll(mcr):
+    ps = stream(8, 24, paths=(1_000, 30_000))
+    results = [None] * len(ps)
+    with mcr.Server(round_ms=1.0) as srv:
+        def client(idx):
+            for i in idx:
+                t = srv.submit(ps[i])
+                while True:
+                    got = srv.poll(t)
+                    if got is not None:
+                        results[i] = got[0]
+                        break
+                    time.sleep(0.0005)
+        threads = [threading.Thread(target=client, args=(range(c, len(ps), 4),)) for c in range(4)]
+        for th in threads:
+            th.start()
+        for th in threads:
+            th.join(timeout=120)
+        assert srv.stats().completed == len(ps)
+    for p, r in zip(ps, results):
+        same(r, alone(mcr, p))
+
+
+def test_rejects_unservable_queries(mcr):
+    p = stream(9, 1, sizes=(10,))[0]
+    with mcr.Server() as srv:
+        for bad in ({"precision": mcr.PREC_FP32}, {"extra_flags": mcr.F_NEAR}):
+            q = dict(p, **bad)
+            with pytest.raises(mcr.McrError) as e:
+                srv.submit(q)
+            assert e.value.code == mcr.MCR_E_UNSUPPORTED
