@@ -20,6 +20,10 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP32_TFLOPS = 157.3    # MI355X published fp32
 PEAK_BF16_TFLOPS = 2500.0   # MI355X published dense bf16 MFMA
+# HBM bytes per launch of the workload's path kernel, from two rocprofv3 --pmc
+# runs of this bench (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 KiB, the factors
+# read off a 256 MiB torch.neg calibration kernel in the same runs).
+TRAFFIC_BYTES = {"mfma": (226.00 * 2.000 + 1410.72) * 1024}
 
 WORKLOAD = {"name": "fused_n50_100M", "n": 50, "factors": 5, "days": 2000, "paths": 100_000_000,
             "k": 10, "x": 0.02, "alpha": 0.99, "data_seed": 1002, "path_seed": 20251202}
@@ -35,8 +39,8 @@ LARGE = {"name": "large_n600_10M", "n": 600, "factors": 20, "days": 2500, "paths
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=10)
-    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=8)
     p.add_argument("--precision", default="auto", choices=["auto", "fp32", "bf16x3"])
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -255,9 +259,9 @@ def main():
 
         src = torch.ones(args.copy_mb << 18, dtype=torch.float32, device="cuda:0")
         dst = torch.empty_like(src)
-        dst.copy_(src)
+        torch.neg(src, out=dst)  # one elementwise kernel: copy_mb MiB read, copy_mb MiB written
         torch.cuda.synchronize()
-        print(f"# calibration copy: {args.copy_mb} MiB read + {args.copy_mb} MiB written",
+        print(f"# calibration kernel (neg): {args.copy_mb} MiB read + {args.copy_mb} MiB written",
               file=sys.stderr, flush=True)
     W = WORKLOAD
     n = W["n"]
@@ -304,7 +308,7 @@ def main():
                    "x": W["x"], "var_alpha": W["alpha"], "kernel": r.kernel,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None,
+                     "frac": achieved / peak, "traffic": TRAFFIC_BYTES.get(r.kernel),
                      "kernel": r.kernel, "kernel_ms": k_ms,
                      "flop_per_path": n * (n + 1)},
         "result": {"count": r.count, "p_hat": r.p_hat, "var99": r.var[0], "es99": r.es[0]},

@@ -18,6 +18,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <vector>
 
@@ -105,8 +106,6 @@ int reserve_host(size_t bytes) {
   g_ctx.host_bytes = want;
   return MCR_OK;
 }
-
-// Carves 256-byte aligned pieces out of a base pointer (device or host).
 
 // Inverse standard normal cdf by Newton on erfc from a bisection bracket.
 double inv_norm_cdf(double p) {
@@ -446,8 +445,11 @@ int resolve_quantile(int64_t M, const DevAccum& acc, int slot, const float* gath
     if (hipMemcpy(g.data(), gather, (size_t)G * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
       return -1;
   }
-  std::sort(g.begin(), g.end());
+  // The (idx+1)-th smallest in-window loss is the quantile; the values after
+  // it in the partition are exactly the larger ones, and their fixed-point
+  // sum does not depend on order, so a partition gives what a sort would.
   const int64_t idx = R - below - 1;
+  std::nth_element(g.begin(), g.begin() + idx, g.end());
   int64_t t = acc.sum_above[slot];
   for (int64_t i = idx; i < G; ++i) t += fixed32_host(g[(size_t)i]);
   *var = (double)g[(size_t)idx];
@@ -657,6 +659,71 @@ void fill_statistics(int64_t M, const DevInfo& info, const DevAccum& acc, mcr_re
 // persistent launch of k_batch_paths over (query, 64-path group run) work
 // items, optionally captured once in a hipGraph and replayed.
 // ---------------------------------------------------------------------------
+BatchKernel class_kernel(int n) {
+  const int T = (n + 1 + 31) / 32;
+  return n <= kSmallMaxN ? batch_small_kernel(n, T) : batch_big_kernel(T);
+}
+
+int class_grid(BatchKernel k) {
+  static std::mutex mu;
+  static std::map<BatchKernel, int> grids;
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = grids.find(k);
+  if (it != grids.end()) return it->second;
+  int cus = g_ctx.num_cus;
+  if (cus <= 0) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      cus = 1;
+  }
+  const int g = batch_kernel_grid(k, cus);
+  grids[k] = g;
+  return g;
+}
+
+void order_by_class(std::vector<WorkItem>* items, const std::vector<BatchKernel>& qk,
+                    const std::vector<double>& qcost, bool lpt, std::vector<ClassRun>* runs) {
+  // classes by descending total cost, then items longest first (or as given)
+  std::map<BatchKernel, double> ccost;
+  for (const WorkItem& w : *items) ccost[qk[w.query]] += qcost[w.query] * w.ngroups;
+  std::vector<std::pair<double, BatchKernel>> order;
+  for (auto& kv : ccost) order.push_back({kv.second, kv.first});
+  std::sort(order.begin(), order.end(),
+            [](const std::pair<double, BatchKernel>& a, const std::pair<double, BatchKernel>& b) {
+              return a.first > b.first;
+            });
+  std::map<BatchKernel, int> rank;
+  for (size_t i = 0; i < order.size(); ++i) rank[order[i].second] = (int)i;
+  std::stable_sort(items->begin(), items->end(), [&](const WorkItem& x, const WorkItem& y) {
+    const int rx = rank[qk[x.query]], ry = rank[qk[y.query]];
+    if (rx != ry) return rx < ry;
+    return lpt && qcost[x.query] * x.ngroups > qcost[y.query] * y.ngroups;
+  });
+  runs->clear();
+  for (size_t i = 0; i < items->size(); ++i) {
+    const BatchKernel k = qk[(*items)[i].query];
+    if (runs->empty() || runs->back().k != k) runs->push_back({k, (uint32_t)i, 0u, class_grid(k)});
+    runs->back().count++;
+  }
+}
+
+int launch_runs(const std::vector<ClassRun>& runs, const PathArgs* qs, const WorkItem* items,
+                unsigned* next, int grid_override, hipStream_t stream) {
+  if (runs.empty()) return MCR_OK;
+  HIP_TRY(launch_batch_clear(next, (int)runs.size(), stream));
+  for (size_t c = 0; c < runs.size(); ++c) {
+    BatchArgs b;
+    b.qs = qs;
+    b.items = items + runs[c].off;
+    b.nitems = runs[c].count;
+    b.next = next + c;
+    HIP_TRY(launch_batch_kernel(runs[c].k, b, grid_override > 0 ? grid_override : runs[c].grid,
+                                stream));
+  }
+  return MCR_OK;
+}
+
 struct Tuned {
   int chunk_paths = 2048;
   int order = MCR_ORDER_LPT;
@@ -686,6 +753,7 @@ struct BatchPlan {
   int nq = 0, max_n = 0;
   std::vector<int> T, S, na;
   std::vector<WorkItem> items;
+  std::vector<ClassRun> runs;  // contiguous item ranges, one persistent launch each
   // device regions
   size_t in_bytes = 0, out_bytes = 0, total = 0;
   // offsets within the input region
@@ -734,26 +802,31 @@ int plan_batch(const mcr_query* qs, int nq, int chunk_paths, int order, BatchPla
     bp->na[i] = (q->flags & MCR_F_VAR) ? q->n_alpha : 0;
     bp->max_n = std::max(bp->max_n, q->n);
   }
-  // work items
+  // work items: per class, runs of cg groups, with at least four items per
+  // resident workgroup (small classes: per wave) so the class's launch balances
+  std::vector<BatchKernel> qk(nq);
+  std::vector<double> qcost(nq);
+  std::map<BatchKernel, uint64_t> class_groups;
+  for (int i = 0; i < nq; ++i) {
+    qk[i] = class_kernel(qs[i].n);
+    qcost[i] = group_cost(bp->T[i], bp->S[i]);
+    class_groups[qk[i]] += ((uint64_t)qs[i].paths + 63) / 64;
+  }
   const uint64_t cg = (uint64_t)std::max(1, chunk_paths / 64);
   bp->items.clear();
   for (int i = 0; i < nq; ++i) {
+    const uint64_t slots = (uint64_t)class_grid(qk[i]) * (qs[i].n <= kSmallMaxN ? 4 : 1);
+    const uint64_t cgi = std::max<uint64_t>(1, std::min(cg, class_groups[qk[i]] / (4 * slots)));
     const uint64_t G = ((uint64_t)qs[i].paths + 63) / 64;
-    for (uint64_t f = 0; f < G; f += cg) {
+    for (uint64_t f = 0; f < G; f += cgi) {
       WorkItem w;
       w.query = (uint32_t)i;
       w.first_group = f;
-      w.ngroups = (uint32_t)std::min<uint64_t>(cg, G - f);
+      w.ngroups = (uint32_t)std::min<uint64_t>(cgi, G - f);
       bp->items.push_back(w);
     }
   }
-  if (order == MCR_ORDER_LPT) {
-    std::vector<double> cost(nq);
-    for (int i = 0; i < nq; ++i) cost[i] = group_cost(bp->T[i], bp->S[i]);
-    std::stable_sort(bp->items.begin(), bp->items.end(), [&](const WorkItem& x, const WorkItem& y) {
-      return cost[x.query] * x.ngroups > cost[y.query] * y.ngroups;
-    });
-  }
+  order_by_class(&bp->items, qk, qcost, order == MCR_ORDER_LPT, &bp->runs);
   // layout: [inputs | outputs | scratch]
   Carver c{nullptr};
   bp->offA.resize(nq);
@@ -801,7 +874,7 @@ int plan_batch(const mcr_query* qs, int nq, int chunk_paths, int order, BatchPla
     c.take<float>((size_t)std::max(bp->na[i], 1) * kBatchGatherCap);
   }
   bp->offNext = c.off;
-  c.take<unsigned>(1);
+  c.take<unsigned>(bp->runs.size());
   bp->offHist = c.off;
   c.take<unsigned>(kHistBins);
   bp->total = c.off;
@@ -896,15 +969,11 @@ uint64_t batch_signature(const mcr_query* qs, const BatchPlan& bp, int grid, int
 int enqueue_batch(const BatchPlan& bp, char* dev, char* hout, int grid) {
   const FactorArgs* fa = (const FactorArgs*)(dev + bp.offFa);
   const PrepArgs* pa = (const PrepArgs*)(dev + bp.offPa);
-  BatchArgs b;
-  b.qs = (const PathArgs*)(dev + bp.offQa);
-  b.items = (const WorkItem*)(dev + bp.offItems);
-  b.nitems = (uint32_t)bp.items.size();
-  b.next = (unsigned*)(dev + bp.offNext);
   HIP_TRY(launch_factor_batch(fa, bp.nq, g_ctx.stream));
   HIP_TRY(launch_prep_batch(pa, bp.nq, bp.max_n, g_ctx.stream));
-  HIP_TRY(launch_batch_clear(b, g_ctx.stream));
-  HIP_TRY(launch_batch_paths(b, grid, g_ctx.stream));
+  MCR_TRY(launch_runs(bp.runs, (const PathArgs*)(dev + bp.offQa),
+                      (const WorkItem*)(dev + bp.offItems), (unsigned*)(dev + bp.offNext), grid,
+                      g_ctx.stream));
   HIP_TRY(hipMemcpyAsync(hout, dev + bp.offInfo, bp.out_bytes, hipMemcpyDeviceToHost, g_ctx.stream));
   return MCR_OK;
 }
@@ -941,7 +1010,7 @@ int run_batch_locked(const mcr_query* qs, int nq, mcr_result* rs, mcr_batch_opts
   }
   BatchPlan bp;
   MCR_TRY(plan_batch(qs, nq, chunk, order, &bp));
-  const int grid = o->grid > 0 ? o->grid : g_ctx.num_cus;
+  const int grid = o->grid;  // 0: each class kernel at its own occupancy
   MCR_TRY(reserve_arena(bp.total));
   MCR_TRY(reserve_host(bp.in_bytes + bp.out_bytes));
   char* dev = (char*)g_ctx.arena;
@@ -999,7 +1068,7 @@ int run_batch_locked(const mcr_query* qs, int nq, mcr_result* rs, mcr_batch_opts
     }
     const int64_t M = q->paths;
     fill_statistics(M, infos[i], accs[i], r);
-    r->kernel = MCR_KERNEL_MFMA_LDS;
+    r->kernel = q->n <= kSmallMaxN ? MCR_KERNEL_MFMA : MCR_KERNEL_MFMA_LDS;
     r->ms_kernel = ms;
     r->var_passes = bp.na[i] ? 1 : 0;
     bool ok = true;
@@ -1017,10 +1086,10 @@ int run_batch_locked(const mcr_query* qs, int nq, mcr_result* rs, mcr_batch_opts
     if (!ok) refine.push_back(i);
   }
   // Rare: a quantile missed its window.  Recompute that query alone on the
-  // same kernel family (identical per-path losses) with the refinement passes.
+  // same kernel (identical per-path losses) with the refinement passes.
   for (int i : refine) {
     mcr_query q = qs[i];
-    q.kernel = MCR_KERNEL_MFMA_LDS;
+    q.kernel = MCR_KERNEL_AUTO;
     q.precision = MCR_PREC_BF16X3;
     mcr_result one;
     MCR_TRY(run_locked(&q, &one));

@@ -5,7 +5,7 @@
 //   Z ~ N(0, I_N), R = mu + L Z (lower-triangular, :51-57), S = #{R_i < -x} (:60-65),
 //   extreme iff S >= k (:69-71), and stores S (:67).
 // Here:
-//   k_factor     FP64 Cholesky-Crout on one workgroup with the jitter policy, then
+//   k_factor     FP64 blocked Cholesky on one workgroup with the jitter policy, then
 //                the portfolio loading v = L^T w, w^T mu and ||v|| (one launch).
 //   k_prep       fp32 / bf16 layouts of the factor for the path kernels.
 //   k_valu_reg   fused path kernel, one path per lane, n <= 64 rows in registers,
@@ -24,9 +24,8 @@
 namespace mcr {
 
 // ---------------------------------------------------------------------------
-// Factor: FP64 Cholesky-Crout of A + delta I on one workgroup of 16 waves.
-// Column j: every row i >= j forms A_ij - sum_{k<j} L_ik L_jk (a wave per row,
-// lanes striding k, DPP reduction), then the pivot test, sqrt and scaling.
+// Factor: FP64 Cholesky of A + delta I on one workgroup of 16 waves (blocked,
+// right-looking; see cholesky_body), then the portfolio moments.
 // Jitter policy: delta_0 = 0, delta_t = mean(diag) 10^(t-12) for t = 1..8, then
 // the modified factorisation with failing pivots clamped to delta_8.  A pivot
 // fails when it is not above n eps (A_jj + delta).
@@ -43,25 +42,42 @@ __device__ double block_sum_f64(double v, double* red) {
   return s;
 }
 
-// FP64 Crout factorisation of A into L with diagonal jitter: attempt t adds
-// delta_t = mean(diag A) * 10^(min(t,8)-12) (t >= 1); attempt 9 clamps failing
-// pivots to delta.  Returns false (and records attempts = -1) for non-finite A.
-__device__ bool cholesky_body(const FactorArgs& a, double* col, double* red, int* attempts_out,
+// FP64 Cholesky of A + delta I into L (row-major, zero upper triangle) on one
+// workgroup of 1024 threads, right-looking by panels of kPanel columns:
+//   panel:    thread r holds row k0 + r of the panel in registers; column c is
+//             finished with one barrier: rows c..kb-1 publish their column-c
+//             entry (double-buffered LDS slot), every row tests the same
+//             pivot, scales its entry and updates its later panel columns;
+//   trailing: A22 -= L21 L21^T on the lower triangle, 4 x 4 register tiles
+//             read from a column-major LDS copy of the panel.
+// Jitter policy: attempt t adds delta_t = mean(diag A) * 10^(min(t,8)-12)
+// (t >= 1); attempt 9 clamps failing pivots to delta.  A pivot fails when it
+// is not above n eps (A_jj + delta).  Returns false (and records attempts =
+// -1) for non-finite A.
+constexpr int kPanel = 16;
+constexpr int kPanelLd = (kMaxN + 3) / 4 * 4 + 4;  // rows + tile overrun, multiple of 4
+
+struct FactorShared {
+  __attribute__((aligned(16))) double panel[kPanel * kPanelLd];  // [c * kPanelLd + r]
+  double pivot[2][kPanel];
+  double adiag[kPanel];  // A_jj + delta of the panel's columns (pivot tolerance)
+  double red[16];
+};
+
+__device__ bool cholesky_body(const FactorArgs& a, FactorShared& sh, int* attempts_out,
                               int* clamped_out, int* fail_out, double* delta_out) {
   const int n = a.n;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, nthr = blockDim.x;  // nthr >= kMaxN: every panel row has an owner
+  const int lane = tid & 63, wave = tid >> 6, nwave = nthr >> 6;
   const double* __restrict__ A = a.A;
   double* __restrict__ L = a.L;
 
   double bad = 0.0, dsum = 0.0;
-  for (int e = tid; e < n * n; e += blockDim.x) {
-    const double v = A[e];
-    if (!isfinite(v)) bad = 1.0;
-    L[e] = 0.0;
-  }
+  for (int e = tid; e < n * n; e += nthr)
+    if (!isfinite(A[e])) bad = 1.0;
   if (tid < n) dsum = A[tid * n + tid];
-  bad = block_sum_f64(bad, red);
-  dsum = block_sum_f64(dsum, red);
+  bad = block_sum_f64(bad, sh.red);
+  dsum = block_sum_f64(dsum, sh.red);
   if (bad > 0.0) {
     if (tid == 0) {
       a.info->attempts = -1;
@@ -81,31 +97,108 @@ __device__ bool cholesky_body(const FactorArgs& a, double* col, double* red, int
     delta = (t == 0) ? 0.0 : mean * pow(10.0, (double)((t < 8 ? t : 8) - 12));
     fail = 0;
     clamped = 0;
-    for (int j = 0; j < n && !fail; ++j) {
-      const double* Lj = L + (size_t)j * n;
-      for (int i = j + wave; i < n; i += 16) {
-        const double* Li = L + (size_t)i * n;
-        double p = 0.0;
-        for (int kk = lane; kk < j; kk += 64) p += Li[kk] * Lj[kk];
-        p = wave_sum_f64(p);
-        if (lane == 0) col[i] = A[(size_t)i * n + j] - p;
+    for (int i = wave; i < n; i += nwave)
+      for (int j = lane; j < n; j += 64) {
+        const size_t e = (size_t)i * n + j;
+        L[e] = (j < i) ? A[e] : (j == i ? A[e] + delta : 0.0);
       }
-      __syncthreads();
-      const double ajj = A[(size_t)j * n + j] + delta;
-      double d = col[j] + delta;
-      const double tol = (double)n * DBL_EPSILON * ajj;
-      if (!(d > tol) || !isfinite(d)) {
-        if (clamp && isfinite(d)) {
-          d = delta;
-          ++clamped;
-        } else {
-          fail = j + 1;
+    __syncthreads();
+
+    for (int k0 = 0; k0 < n && !fail; k0 += kPanel) {
+      const int kb = min(kPanel, n - k0), rows = n - k0;
+      const int r = tid;
+      const bool own = r < rows;
+      double x[kPanel];
+#pragma unroll
+      for (int c = 0; c < kPanel; ++c)
+        x[c] = (own && c < kb) ? L[(size_t)(k0 + r) * n + k0 + c] : 0.0;
+      if (tid < kb) sh.adiag[tid] = A[(size_t)(k0 + tid) * n + k0 + tid] + delta;
+
+#pragma unroll
+      for (int c = 0; c < kPanel; ++c) {
+        const bool live = c < kb && !fail;  // uniform over the workgroup
+        if (live && own && r >= c && r < kb) sh.pivot[c & 1][r] = x[c];
+        __syncthreads();
+        if (live) {
+          const int j = k0 + c;
+          double d = sh.pivot[c & 1][c];
+          const double tol = (double)n * DBL_EPSILON * sh.adiag[c];
+          if (!(d > tol) || !isfinite(d)) {
+            if (clamp && isfinite(d)) {
+              d = delta;
+              ++clamped;
+            } else {
+              fail = j + 1;
+            }
+          }
+          if (!fail) {
+            const double ljj = sqrt(d);
+            const double rl = 1.0 / ljj;
+            if (own && r == c) x[c] = ljj;
+            if (own && r > c) {
+              const double lr = x[c] * rl;
+              x[c] = lr;
+#pragma unroll
+              for (int c2 = c + 1; c2 < kPanel; ++c2)
+                if (c2 < kb && r >= c2) x[c2] = fma(-lr, sh.pivot[c & 1][c2] * rl, x[c2]);
+            }
+          }
         }
       }
-      if (!fail) {
-        const double ljj = sqrt(d);
-        for (int i = j + tid; i < n; i += blockDim.x)
-          L[(size_t)i * n + j] = (i == j) ? ljj : col[i] / ljj;
+      if (fail) break;
+
+      if (own) {
+#pragma unroll
+        for (int c = 0; c < kPanel; ++c) {
+          if (c < kb) L[(size_t)(k0 + r) * n + k0 + c] = x[c];
+          sh.panel[c * kPanelLd + r] = x[c];
+        }
+      }
+      for (int e = tid; e < kPanel * (kPanelLd - rows); e += nthr) {
+        const int c = e / (kPanelLd - rows), q = e % (kPanelLd - rows);
+        sh.panel[c * kPanelLd + rows + q] = 0.0;
+      }
+      __syncthreads();
+
+      // trailing update of columns k1..n-1 (only when the panel was full)
+      const int k1 = k0 + kb, m = n - k1;
+      if (m > 0) {
+        const int TB = (m + 3) >> 2;
+        const int ntiles = TB * (TB + 1) / 2;
+        for (int tt = tid; tt < ntiles; tt += nthr) {
+          int bi = (int)((sqrt(8.0 * tt + 1.0) - 1.0) * 0.5);
+          while (bi * (bi + 1) / 2 > tt) --bi;
+          while ((bi + 1) * (bi + 2) / 2 <= tt) ++bi;
+          const int bj = tt - bi * (bi + 1) / 2;
+          const int ri = kPanel + 4 * bi, rj = kPanel + 4 * bj;
+          double acc[4][4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
+#pragma unroll 1
+          for (int p = 0; p < kPanel; ++p) {
+            const double2* pa = (const double2*)&sh.panel[p * kPanelLd + ri];
+            const double2* pb = (const double2*)&sh.panel[p * kPanelLd + rj];
+            const double2 a01 = pa[0], a23 = pa[1], b01 = pb[0], b23 = pb[1];
+            const double av[4] = {a01.x, a01.y, a23.x, a23.y};
+            const double bv[4] = {b01.x, b01.y, b23.x, b23.y};
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+              for (int v = 0; v < 4; ++v) acc[u][v] = fma(av[u], bv[v], acc[u][v]);
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int i = k1 + 4 * bi + u;
+            if (i >= n) break;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+              const int j = k1 + 4 * bj + v;
+              if (j <= i) L[(size_t)i * n + j] -= acc[u][v];
+            }
+          }
+        }
       }
       __syncthreads();
     }
@@ -165,7 +258,7 @@ __device__ void moments_body(const FactorArgs& a, double* red, int attempts, int
 }
 
 
-__device__ void factor_body(const FactorArgs& a, double* col, double* red) {
+__device__ void factor_body(const FactorArgs& a, FactorShared& sh) {
   const int tid = threadIdx.x;
 
   // Clear the query's accumulators.
@@ -193,7 +286,7 @@ __device__ void factor_body(const FactorArgs& a, double* col, double* red) {
       }
       return;
     }
-  } else if (!cholesky_body(a, col, red, &attempts, &clamped, &last_fail, &delta)) {
+  } else if (!cholesky_body(a, sh, &attempts, &clamped, &last_fail, &delta)) {
     return;
   }
   if (a.finfo && !a.reuse && tid == 0) {
@@ -202,21 +295,19 @@ __device__ void factor_body(const FactorArgs& a, double* col, double* red) {
     a.finfo->fail_col = last_fail;
     a.finfo->jitter = delta;
   }
-  moments_body(a, red, attempts, clamped, last_fail, delta);
+  moments_body(a, sh.red, attempts, clamped, last_fail, delta);
 }
 
 __global__ __launch_bounds__(1024) void k_factor(FactorArgs a) {
-  __shared__ double col[kMaxN];
-  __shared__ double red[16];
-  factor_body(a, col, red);
+  __shared__ FactorShared sh;
+  factor_body(a, sh);
 }
 
 // One workgroup per query of a batch.
 __global__ __launch_bounds__(1024) void k_factor_batch(const FactorArgs* fas) {
-  __shared__ double col[kMaxN];
-  __shared__ double red[16];
+  __shared__ FactorShared sh;
   const FactorArgs a = fas[blockIdx.x];
-  factor_body(a, col, red);
+  factor_body(a, sh);
 }
 
 hipError_t launch_factor(const FactorArgs& a, hipStream_t stream) {

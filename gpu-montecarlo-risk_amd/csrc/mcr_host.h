@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <mutex>
+#include <vector>
 
 #include "../../include/mcr_hip.h"
 #include "mcr_internal.h"
@@ -71,6 +72,22 @@ int resolve_quantile(int64_t M, const DevAccum& acc, int slot, const float* gath
                      int64_t R, double* var, double* es, int64_t* tail);
 int run_locked(const mcr_query* q, mcr_result* r);
 void set_windows(const mcr_query* q, int n_alpha, FactorArgs* f);
+
+// Persistent batch classes: queries with n <= kSmallMaxN run the per-wave
+// register-operand MFMA kernel of their (NB, T), larger ones the LDS MFMA
+// kernel of their TW.  Items of one class are contiguous and launched together.
+constexpr int kSmallMaxN = 127;
+struct ClassRun {
+  BatchKernel k;
+  uint32_t off, count;  // item range
+  int grid;             // resident workgroups of k
+};
+BatchKernel class_kernel(int n);
+int class_grid(BatchKernel k);
+void order_by_class(std::vector<WorkItem>* items, const std::vector<BatchKernel>& qk,
+                    const std::vector<double>& qcost, bool lpt, std::vector<ClassRun>* runs);
+int launch_runs(const std::vector<ClassRun>& runs, const PathArgs* qs, const WorkItem* items,
+                unsigned* next, int grid_override, hipStream_t stream);
 void fill_statistics(int64_t M, const DevInfo& info, const DevAccum& acc, mcr_result* r);
 
 }  // namespace host

@@ -1,4 +1,5 @@
-// mcr_mfma.hip -- the bf16x3 MFMA path kernel (n <= 127).
+// mcr_mfma.hip -- the bf16x3 MFMA path kernel (n <= 127) and its persistent
+// batch form.
 #include "mcr_kern_common.h"
 
 namespace mcr {
@@ -17,18 +18,14 @@ namespace mcr {
 // ---------------------------------------------------------------------------
 // NB = Philox blocks per path (n <= 4 NB), T = row tiles; both compile-time
 // so every block, step and tile bound below is a constant.
+// Groups g0, g0 + gstep, ... < g1 of 64 paths of query a, one group per wave
+// at a time, accumulated into the lane tally t (no workgroup barriers).
 template <int NB, int T>
-__global__ __launch_bounds__(256) void k_mfma(PathArgs a) {
+__device__ __forceinline__ void mfma_wave_groups(const PathArgs& a, uint64_t g0, uint64_t g1,
+                                                 uint64_t gstep, LaneTally& t, unsigned* sh_hist) {
   constexpr int S = (NB + 3) / 4;  // 16-wide K steps
-  __shared__ unsigned sh_hist[32 * kMfmaMaxT];
   const int lane = threadIdx.x & 63;
   const int g = lane >> 5;
-  const int n = a.n;
-  const bool want_hist = a.s_hist != nullptr;
-  if (want_hist) {
-    for (int i = threadIdx.x; i <= n; i += blockDim.x) sh_hist[i] = 0u;
-    __syncthreads();
-  }
   const Key2 key = key_of(a.seed);
   // Factor fragments are streamed per step from L1/L2 (16 B per lane per
   // tile and part) instead of being held for the whole kernel.
@@ -37,11 +34,7 @@ __global__ __launch_bounds__(256) void k_mfma(PathArgs a) {
   // tile r = 15 in the upper half-wave, is excluded explicitly).
   const float4* __restrict__ rinit = (const float4*)a.rowinit;
 
-  LaneTally t;
-  zero_tally(t);
-  const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x >> 6);
-  for (uint64_t grp = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-       grp * 64 < a.npaths; grp += waves) {
+  for (uint64_t grp = g0; grp < g1; grp += gstep) {
     const uint64_t q = grp * 64 + lane;
     const uint64_t path = a.path0 + q;
     f32x16 accA[T], accB[T];
@@ -121,13 +114,49 @@ __global__ __launch_bounds__(256) void k_mfma(PathArgs a) {
     uint32_t la = as_u32(accA[T - 1][15]), lb = as_u32(accB[T - 1][15]);
     swap32(la, lb);
     const float loss = as_f32(lb);
-    if (q < a.npaths) tally_path(a, t, q, Sp, loss, false, want_hist ? sh_hist : nullptr);
+    if (q < a.npaths) tally_path(a, t, q, Sp, loss, false, sh_hist);
   }
+}
+
+template <int NB, int T>
+__global__ __launch_bounds__(256) void k_mfma(PathArgs a) {
+  __shared__ unsigned sh_hist[32 * kMfmaMaxT];
+  const int n = a.n;
+  const bool want_hist = a.s_hist != nullptr;
+  if (want_hist) {
+    for (int i = threadIdx.x; i <= n; i += blockDim.x) sh_hist[i] = 0u;
+    __syncthreads();
+  }
+  LaneTally t;
+  zero_tally(t);
+  const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x >> 6);
+  mfma_wave_groups<NB, T>(a, (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6),
+                          (a.npaths + 63) / 64, waves, t, want_hist ? sh_hist : nullptr);
   flush_tally(a, t);
   if (want_hist) {
     __syncthreads();
     for (int i = threadIdx.x; i <= n; i += blockDim.x)
       if (sh_hist[i]) atomicAdd(&a.s_hist[i], (unsigned long long)sh_hist[i]);
+  }
+}
+
+// Persistent batch kernel for one (NB, T) class: every wave takes work items
+// (a run of 64-path groups of one query) from the class's atomic counter until
+// the list is exhausted, and flushes its tally into the item's query after
+// each item.  Waves never wait for each other.
+template <int NB, int T>
+__global__ __launch_bounds__(256) void k_batch_small(BatchArgs b) {
+  for (;;) {
+    unsigned it = 0;
+    if ((threadIdx.x & 63) == 0) it = atomicAdd(b.next, 1u);
+    it = __builtin_amdgcn_readfirstlane(it);  // lane 0 took the item for the wave
+    if (it >= b.nitems) break;
+    const WorkItem w = load_const(b.items + it);
+    const PathArgs a = load_const(b.qs + w.query);
+    LaneTally t;
+    zero_tally(t);
+    mfma_wave_groups<NB, T>(a, w.first_group, w.first_group + w.ngroups, 1, t, nullptr);
+    flush_tally(a, t);
   }
 }
 
@@ -140,43 +169,57 @@ __global__ __launch_bounds__(256) void k_mfma(PathArgs a) {
 constexpr int kNbLo = 8 * MCR_MFMA_PART + 1;
 constexpr int kNbHi = 8 * MCR_MFMA_PART + 8;
 
+struct MfmaTables {
+  PathKernel path[kMfmaNB + 1][kMfmaMaxT + 1];
+  BatchKernel batch[kMfmaNB + 1][kMfmaMaxT + 1];
+};
+
 template <int NB>
 struct MfmaFill {
-  static void fill(PathKernel (*t)[kMfmaMaxT + 1]) {
+  static void fill(MfmaTables* t) {
     constexpr int Tlo = (4 * NB - 2 + 31) / 32;  // n = 4 NB - 3
     constexpr int Thi = (4 * NB + 1 + 31) / 32;  // n = 4 NB
-    t[NB][Tlo] = k_mfma<NB, Tlo>;
-    if constexpr (Thi != Tlo && Thi <= kMfmaMaxT) t[NB][Thi] = k_mfma<NB, Thi>;
+    t->path[NB][Tlo] = k_mfma<NB, Tlo>;
+    t->batch[NB][Tlo] = k_batch_small<NB, Tlo>;
+    if constexpr (Thi != Tlo && Thi <= kMfmaMaxT) {
+      t->path[NB][Thi] = k_mfma<NB, Thi>;
+      t->batch[NB][Thi] = k_batch_small<NB, Thi>;
+    }
     if constexpr (NB < kNbHi) MfmaFill<NB + 1>::fill(t);
   }
 };
 
 #define MCR_CAT2(a, b) a##b
 #define MCR_CAT(a, b) MCR_CAT2(a, b)
-void MCR_CAT(mfma_fill_part, MCR_MFMA_PART)(PathKernel (*t)[kMfmaMaxT + 1]) {
-  MfmaFill<kNbLo>::fill(t);
-}
+void MCR_CAT(mfma_fill_part, MCR_MFMA_PART)(MfmaTables* t) { MfmaFill<kNbLo>::fill(t); }
 
 #if MCR_MFMA_PART == 0
-void mfma_fill_part1(PathKernel (*t)[kMfmaMaxT + 1]);
-void mfma_fill_part2(PathKernel (*t)[kMfmaMaxT + 1]);
-void mfma_fill_part3(PathKernel (*t)[kMfmaMaxT + 1]);
+void mfma_fill_part1(MfmaTables* t);
+void mfma_fill_part2(MfmaTables* t);
+void mfma_fill_part3(MfmaTables* t);
+
+static const MfmaTables& mfma_tables() {
+  static const MfmaTables tables = [] {
+    MfmaTables t = {};
+    mfma_fill_part0(&t);
+    mfma_fill_part1(&t);
+    mfma_fill_part2(&t);
+    mfma_fill_part3(&t);
+    return t;
+  }();
+  return tables;
+}
 
 PathKernel mfma_kernel(int n, int T) {
-  static PathKernel table[kMfmaNB + 1][kMfmaMaxT + 1];
-  static bool init = false;
-  if (!init) {
-    for (auto& row : table)
-      for (auto& k : row) k = nullptr;
-    mfma_fill_part0(table);
-    mfma_fill_part1(table);
-    mfma_fill_part2(table);
-    mfma_fill_part3(table);
-    init = true;
-  }
   const int nb = (n + 3) / 4;
   if (nb < 1 || nb > kMfmaNB || T < 1 || T > kMfmaMaxT) return nullptr;
-  return table[nb][T];
+  return mfma_tables().path[nb][T];
+}
+
+BatchKernel batch_small_kernel(int n, int T) {
+  const int nb = (n + 3) / 4;
+  if (nb < 1 || nb > kMfmaNB || T < 1 || T > kMfmaMaxT) return nullptr;
+  return mfma_tables().batch[nb][T];
 }
 #endif
 

@@ -171,12 +171,12 @@ __global__ __launch_bounds__(256) void k_mfma_big(PathArgs a) {
   }
 }
 
-// Persistent batch kernel: every workgroup takes work items (a run of 64-path
-// groups of one query) from an atomic counter until the list is exhausted;
-// the exit test is uniform, so every wave of every workgroup leaves the loop.
-// Tallies are flushed into the item's query after each item.
-constexpr int kBatchTW = (kMaxN + 1 + 127) / 128;
-
+// Persistent batch kernel for one TW class: every workgroup takes work items
+// (a run of 64-path groups of one query) from the class's atomic counter until
+// the list is exhausted; the exit test is uniform, so every wave of every
+// workgroup leaves the loop.  Tallies are flushed into the item's query after
+// each item.
+template <int TW>
 __global__ __launch_bounds__(256) void k_batch_paths(BatchArgs b) {
   __shared__ BigShared sh;
   __shared__ unsigned s_item;
@@ -192,23 +192,45 @@ __global__ __launch_bounds__(256) void k_batch_paths(BatchArgs b) {
     const PathArgs a = load_const(b.qs + w.query);
     LaneTally t;
     zero_tally(t);
-    mfma_lds_groups<kBatchTW>(a, w.first_group, w.first_group + w.ngroups, 1, sh, t, nullptr);
+    mfma_lds_groups<TW>(a, w.first_group, w.first_group + w.ngroups, 1, sh, t, nullptr);
     flush_tally(a, t);
   }
 }
 
-__global__ void k_batch_clear(BatchArgs b) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *b.next = 0u;
+__global__ void k_batch_clear(unsigned* next, int count) {
+  for (int i = threadIdx.x; i < count; i += blockDim.x) next[i] = 0u;
 }
 
-hipError_t launch_batch_clear(const BatchArgs& b, hipStream_t stream) {
-  hipLaunchKernelGGL(k_batch_clear, dim3(1), dim3(64), 0, stream, b);
+hipError_t launch_batch_clear(unsigned* next, int count, hipStream_t stream) {
+  hipLaunchKernelGGL(k_batch_clear, dim3(1), dim3(64), 0, stream, next, count);
   return hipGetLastError();
 }
 
-hipError_t launch_batch_paths(const BatchArgs& b, int grid, hipStream_t stream) {
-  hipLaunchKernelGGL(k_batch_paths, dim3(grid), dim3(256), 0, stream, b);
+hipError_t launch_batch_kernel(BatchKernel k, const BatchArgs& b, int grid, hipStream_t stream) {
+  hipLaunchKernelGGL(k, dim3(grid), dim3(256), 0, stream, b);
   return hipGetLastError();
+}
+
+int batch_kernel_grid(BatchKernel k, int num_cus) {
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(k), 256,
+                                                   0) != hipSuccess ||
+      per_cu < 1) {
+    (void)hipGetLastError();
+    per_cu = 1;
+  }
+  return per_cu * num_cus;
+}
+
+BatchKernel batch_big_kernel(int T) {
+  switch ((T + 3) / 4) {
+    case 1: return k_batch_paths<1>;
+    case 2: return k_batch_paths<2>;
+    case 3: return k_batch_paths<3>;
+    case 4: return k_batch_paths<4>;
+    case 5: return k_batch_paths<5>;
+    default: return nullptr;
+  }
 }
 
 PathKernel mfma_big_kernel(int T) {

@@ -1,6 +1,7 @@
 // mcr_server.cpp -- serving loop: risk queries submitted from any thread are
 // admitted by device-memory footprint and run in rounds of the persistent
-// path kernel (k_batch_paths), new queries joining between rounds.
+// path kernels (one launch per kernel class), new queries joining between
+// rounds.
 //
 // The reference runs one query per process (src/main_runner.c:198-296: read
 // inputs, call the model, write a CSV row); nothing there batches or serves.
@@ -11,11 +12,12 @@
 // its paths shared the device with.
 //
 // Device work per round r (parity p = r & 1), on three default-priority streams:
-//   setup:  H2D of new queries' inputs and descriptors, k_factor_batch (Crout
-//           skipped on a cache hit), k_prep_batch, H2D of the round's work
+//   setup:  H2D of new queries' inputs and descriptors, k_factor_batch
+//           (factorisation skipped on a cache hit), k_prep_batch, H2D of the round's work
 //           items; event setup[p].  Overlaps the previous round's path kernel.
-//   paths:  wait setup[p]; k_batch_clear; k_batch_paths over the round's items;
-//           events start[p], end[p].
+//   paths:  wait setup[p]; k_batch_clear; one persistent launch per kernel class
+//           over the round's items (k_batch_small / k_batch_paths); events
+//           start[p], end[p].
 //   out:    wait end[p]; D2H of the records of queries whose last paths ran in
 //           round r; event out[p].
 // The worker enqueues round r, then waits for round r-1's outputs and
@@ -58,6 +60,7 @@ constexpr int kDefaultCache = 256;
 constexpr int kDefaultActive = 1024;
 constexpr double kDefaultRoundMs = 4.0;
 constexpr uint32_t kMaxItems = 1u << 16;  // work items per round
+constexpr int kMaxRuns = 64;              // kernel classes per round (at most 41 exist)
 constexpr int kMaxBypass = 16;            // rounds a blocked head query may be overtaken
 constexpr double kInitMsPerUnit = 7e-5;   // initial device ms per cost unit per workgroup
 
@@ -194,6 +197,7 @@ struct RoundRec {
   std::vector<Job*> members, finishing;
   double units = 0.0;
   size_t nitems = 0;
+  std::vector<ClassRun> runs;
 };
 
 uint64_t hash_sigma(int n, const double* s) {
@@ -243,7 +247,8 @@ struct mcr_server {
   int cache_cap = kDefaultCache;  // < 0: no cache
   int max_active = kDefaultActive;
   double round_ms = kDefaultRoundMs;
-  int grid = 0;
+  int grid = 0;           // CUs: the round budget is grid x round_ms of device time
+  int grid_override = 0;  // > 0: every class kernel launched with this many workgroups
 
   // device resources
   hipStream_t s_setup = nullptr, s_paths = nullptr, s_out = nullptr;
@@ -322,7 +327,8 @@ int mcr_server::init(int dev, const mcr_server_opts* o) {
   HIP_TRY(hipSetDevice(dev));
   int cus = 0;
   HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  grid = (o && o->grid > 0) ? o->grid : cus;
+  grid = cus;
+  grid_override = (o && o->grid > 0) ? o->grid : 0;
   HIP_TRY(hipStreamCreateWithFlags(&s_setup, hipStreamNonBlocking));
   HIP_TRY(hipStreamCreateWithFlags(&s_paths, hipStreamNonBlocking));
   HIP_TRY(hipStreamCreateWithFlags(&s_out, hipStreamNonBlocking));
@@ -347,7 +353,7 @@ int mcr_server::init(int dev, const mcr_server_opts* o) {
   off_pa = r.off;
   r.take<PrepArgs>(max_active);
   off_next = r.off;
-  r.take<unsigned>(1);
+  r.take<unsigned>(kMaxRuns);
   const size_t round_bytes = r.off;
   const size_t hist_off = round_base + 2 * round_bytes;
   const size_t fixed_bytes = hist_off + align_up(kHistBins * sizeof(unsigned));
@@ -567,7 +573,7 @@ int mcr_server::plan_round(uint64_t round, RoundRec* rr, std::vector<WorkItem>* 
     est += (size_t)((gr.second + ig - 1) / ig);
   }
   const double scale = est > kMaxItems ? (double)est / kMaxItems + 1.0 : 1.0;
-  std::vector<std::pair<double, WorkItem>> it;
+  std::vector<std::pair<double, WorkItem>> it;  // (cost, item)
   for (auto& gr : grant) {
     Job* j = gr.first;
     const double c = group_cost(j->T, j->S);
@@ -589,11 +595,18 @@ int mcr_server::plan_round(uint64_t round, RoundRec* rr, std::vector<WorkItem>* 
     set_error("round planned %zu work items (limit %u)", it.size(), kMaxItems);
     return MCR_E_DEVICE;
   }
-  std::stable_sort(it.begin(), it.end(),
-                   [](const std::pair<double, WorkItem>& a, const std::pair<double, WorkItem>& b) {
-                     return a.first > b.first;
-                   });
   for (auto& x : it) items->push_back(x.second);
+  std::vector<BatchKernel> qk(max_active, nullptr);
+  std::vector<double> qcost(max_active, 0.0);
+  for (auto& gr : grant) {
+    qk[gr.first->slot] = class_kernel(gr.first->q.n);
+    qcost[gr.first->slot] = group_cost(gr.first->T, gr.first->S);
+  }
+  order_by_class(items, qk, qcost, true, &rr->runs);
+  if (rr->runs.size() > (size_t)kMaxRuns) {
+    set_error("round needs %zu kernel classes (limit %d)", rr->runs.size(), kMaxRuns);
+    return MCR_E_DEVICE;
+  }
   rr->units = total_units;
   rr->nitems = items->size();
   return MCR_OK;
@@ -703,17 +716,10 @@ int mcr_server::enqueue_round(RoundRec& rr, const std::vector<Job*>& fresh,
                            s_setup));
   HIP_TRY(hipEventRecord(ev_setup[p], s_setup));
 
-  BatchArgs b;
-  b.qs = d_slots;
-  b.items = (const WorkItem*)dr;
-  b.nitems = (uint32_t)items.size();
-  b.next = (unsigned*)(dr + off_next);
   HIP_TRY(hipStreamWaitEvent(s_paths, ev_setup[p], 0));
   HIP_TRY(hipEventRecord(ev_start[p], s_paths));
-  if (!items.empty()) {
-    HIP_TRY(launch_batch_clear(b, s_paths));
-    HIP_TRY(launch_batch_paths(b, grid, s_paths));
-  }
+  MCR_TRY(launch_runs(rr.runs, d_slots, (const WorkItem*)dr, (unsigned*)(dr + off_next),
+                      grid_override, s_paths));
   HIP_TRY(hipEventRecord(ev_end[p], s_paths));
 
   const size_t span = align_up(sizeof(DevInfo)) + sizeof(DevAccum);
@@ -790,7 +796,7 @@ int mcr_server::finalize_round(RoundRec& rr) {
     }
     const int64_t M = j->q.paths;
     fill_statistics(M, info, acc, r);
-    r->kernel = MCR_KERNEL_MFMA_LDS;
+    r->kernel = j->q.n <= kSmallMaxN ? MCR_KERNEL_MFMA : MCR_KERNEL_MFMA_LDS;
     r->var_passes = j->na ? 1 : 0;
     bool ok = true;
     for (int al = 0; al < j->na && ok; ++al) {
@@ -809,7 +815,7 @@ int mcr_server::finalize_round(RoundRec& rr) {
       // Rare: a quantile missed its window.  Same kernel family alone, with
       // the refinement passes (identical per-path losses).
       mcr_query q1 = j->q;
-      q1.kernel = MCR_KERNEL_MFMA_LDS;
+      q1.kernel = MCR_KERNEL_AUTO;
       q1.precision = MCR_PREC_BF16X3;
       mcr_result one;
       {

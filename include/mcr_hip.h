@@ -136,7 +136,8 @@ int mcr_prepare(const mcr_query *q, mcr_model_export *out);
 int mcr_normals(uint64_t seed, uint64_t path0, int64_t npaths, int n, float *out);
 
 /* ---- batches of independent queries (portfolios) ---- */
-#define MCR_BATCH_PERSISTENT 0 /* batched factor + prepare, then ONE persistent path launch */
+#define MCR_BATCH_PERSISTENT 0 /* batched factor + prepare, then one persistent path launch
+                                  per kernel class, all in one hipGraph */
 #define MCR_BATCH_PER_QUERY 1  /* one mcr_run per query (the comparison baseline) */
 #define MCR_ORDER_LPT 0        /* work items by descending cost (longest first) */
 #define MCR_ORDER_NATURAL 1    /* work items in query order */
@@ -147,7 +148,8 @@ typedef struct mcr_batch_opts {
                         layout is unchanged) */
   int chunk_paths;   /* paths per work item, multiple of 64; 0 = autotuned value */
   int order;         /* MCR_ORDER_*; used when chunk_paths != 0 */
-  int grid;          /* persistent workgroups; 0 = one per CU */
+  int grid;          /* persistent workgroups per launch; 0 = each class kernel's
+                        resident maximum (occupancy x CUs) */
   double ms_device;  /* out: device time of the batch (HIP events) */
   double ms_total;   /* out: host wall time of the call */
   int chunk_used;    /* out: chunk_paths actually used */
@@ -158,8 +160,9 @@ typedef struct mcr_batch_opts {
 
 /* Runs nq independent queries.  Persistent mode: every query uses the bf16x3
  * MFMA kernel family (precision MCR_PREC_BF16X3 or MCR_PREC_AUTO; no per-path
- * outputs); results equal those of mcr_run with kernel MCR_KERNEL_MFMA_LDS on
- * the same query.  rs[i] receives query i's result. */
+ * outputs); results equal those of mcr_run on the same query with precision
+ * MCR_PREC_BF16X3 and kernel MCR_KERNEL_AUTO (MCR_KERNEL_MFMA for n <= 127,
+ * MCR_KERNEL_MFMA_LDS above).  rs[i] receives query i's result. */
 int mcr_run_batch(const mcr_query *qs, int nq, mcr_result *rs, mcr_batch_opts *opts);
 
 /* Times candidate (chunk_paths, order) settings on this batch, keeps the
@@ -174,12 +177,13 @@ int mcr_autotune_batch(const mcr_query *qs, int nq, mcr_batch_opts *best, int *n
  * hbm_budget bytes, a cache of FP64 Cholesky factors keyed by covariance
  * content and a worker thread.  Submitted queries are admitted while their
  * device footprint fits the free budget; the worker runs rounds of the
- * persistent path kernel over every admitted query, admitting new queries
+ * persistent path kernels over every admitted query, admitting new queries
  * between rounds, so queries that arrive while others run share the device
  * with them.  Each query's paths are keyed by (seed, path id) and reduced
- * exactly, so its result equals mcr_run with kernel MCR_KERNEL_MFMA_LDS on the
- * same query.  Queries: precision MCR_PREC_BF16X3 or MCR_PREC_AUTO, flags
- * MCR_F_VAR only, no per-path outputs.  All calls are thread-safe. */
+ * exactly, so its result equals mcr_run on the same query with precision
+ * MCR_PREC_BF16X3 and kernel MCR_KERNEL_AUTO.  Queries: precision
+ * MCR_PREC_BF16X3 or MCR_PREC_AUTO, flags MCR_F_VAR only, no per-path outputs.
+ * All calls are thread-safe. */
 typedef struct mcr_server mcr_server;
 
 typedef struct mcr_server_opts {

@@ -1,10 +1,12 @@
 """Batched uneven portfolios: batched factor/prepare + one persistent path
-launch (optionally a replayed hipGraph) against the same queries run alone.
+launch per kernel class (optionally a replayed hipGraph) against the same
+queries run alone.
 
 Bar: every per-query result (count, VaR, tail sum, ES) is identical to the
-query run alone on the same kernel family (MCR_KERNEL_MFMA_LDS): the paths are
-keyed by (seed, path id) and all reductions are integer or exact order
-statistics, so batching cannot change them."""
+query run alone with the same kernel (mcr_run, bf16x3, automatic kernel: the
+register-operand MFMA kernel for n <= 127, the LDS MFMA kernel above): the
+paths are keyed by (seed, path id) and all reductions are integer or exact
+order statistics, so batching cannot change them."""
 from __future__ import annotations
 
 import numpy as np
@@ -30,8 +32,7 @@ def portfolios(seed=0, sizes=(5, 17, 50, 64, 100, 128, 200, 333, 600), paths=Non
 
 def alone(mcr, p, extra_flags=0):
     return mcr.run(p["mu"], p["sigma"], p["x"], p["k"], p["paths"], p["seed"],
-                   weights=p["weights"], precision=mcr.PREC_BF16X3,
-                   kernel=mcr.KERNEL_MFMA_LDS, alphas=p["alphas"], extra_flags=extra_flags)
+                   weights=p["weights"], precision=mcr.PREC_BF16X3, alphas=p["alphas"], extra_flags=extra_flags)
 
 
 def same(a, b):
@@ -47,7 +48,7 @@ def test_batch_equals_alone(mcr):
     assert o.work_items >= len(ps)
     for p, r in zip(ps, res):
         same(r, alone(mcr, p))
-        assert r.kernel == "mfma_lds"
+        assert r.kernel == ("mfma" if len(p["mu"]) <= 127 else "mfma_lds")
 
 
 def test_graph_replay_uses_new_inputs(mcr):

@@ -3,7 +3,7 @@ device-memory footprint, share rounds of the persistent kernel with whatever
 else is running, and reuse cached Cholesky factors.
 
 Bar: every served result (count, VaR, tail sums, ES) is identical to the same
-query run alone on the same kernel family (mcr_run, MCR_KERNEL_MFMA_LDS):
+query run alone with the same kernel (mcr_run, bf16x3, automatic kernel):
 paths are keyed by (seed, path id) and reduced into integer / fixed-point
 tallies, so the mix, the round boundaries and the cache cannot change them."""
 from __future__ import annotations
@@ -43,8 +43,7 @@ def stream(seed, count, sizes=(5, 12, 33, 50, 64, 100, 200, 333, 600), reuse=0.3
 
 def alone(mcr, p):
     return mcr.run(p["mu"], p["sigma"], p["x"], p["k"], p["paths"], p["seed"],
-                   weights=p["weights"], precision=mcr.PREC_BF16X3,
-                   kernel=mcr.KERNEL_MFMA_LDS, alphas=p.get("alphas", ()),
+                   weights=p["weights"], precision=mcr.PREC_BF16X3, alphas=p.get("alphas", ()),
                    extra_flags=p.get("extra_flags", 0))
 
 
@@ -78,7 +77,7 @@ def test_served_results_equal_alone(mcr):
     assert max(info.rounds for _, info in out) > 1  # some query spanned several rounds
     for p, (r, info) in zip(ps, out):
         assert info.status == 0 and info.latency_ms >= info.queue_ms >= 0.0
-        assert r.kernel == "mfma_lds"
+        assert r.kernel == ("mfma" if len(p["mu"]) <= 127 else "mfma_lds")
         same(r, alone(mcr, p))
 
 
