@@ -77,7 +77,8 @@ class Result(ctypes.Structure):
                 ("loss_mean", ctypes.c_double), ("loss_sd", ctypes.c_double),
                 ("jitter", ctypes.c_double), ("factor_attempts", ctypes.c_int),
                 ("factor_clamped", ctypes.c_int), ("kernel", ctypes.c_int),
-                ("ms_kernel", ctypes.c_double), ("ms_total", ctypes.c_double)]
+                ("ms_kernel", ctypes.c_double), ("ms_total", ctypes.c_double),
+                ("ms_device", ctypes.c_double)]
 
 
 class FactorInfo(ctypes.Structure):
@@ -189,6 +190,7 @@ class RiskResult:
     kernel: str = ""
     ms_kernel: float = 0.0
     ms_total: float = 0.0
+    ms_device: float = 0.0
     S: np.ndarray | None = None
     loss: np.ndarray | None = None
     s_hist: np.ndarray | None = None
@@ -244,7 +246,8 @@ def _result(r: Result, na: int, keep=None) -> RiskResult:
                       loss_mean=r.loss_mean, loss_sd=r.loss_sd, jitter=r.jitter,
                       factor_attempts=r.factor_attempts, factor_clamped=r.factor_clamped,
                       kernel=KERNEL_NAMES.get(r.kernel, str(r.kernel)), ms_kernel=r.ms_kernel,
-                      ms_total=r.ms_total, S=S, loss=loss, s_hist=hist)
+                      ms_total=r.ms_total, ms_device=r.ms_device, S=S, loss=loss,
+                      s_hist=hist)
 
 
 def run(mu, sigma, x: float, k: int, paths: int, seed: int = 0, *, weights=None,

@@ -66,6 +66,7 @@ int ensure_init(int device) {
   HIP_TRY(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
   HIP_TRY(hipEventCreate(&g_ctx.ev0));
   HIP_TRY(hipEventCreate(&g_ctx.ev1));
+  HIP_TRY(hipEventCreate(&g_ctx.evs));
   g_ctx.ready = true;
   return MCR_OK;
 }
@@ -324,6 +325,7 @@ int stage_query(const mcr_query* q, Staged* st, bool want_var_windows) {
   fa.info = st->d.info;
   fa.acc = st->d.acc;
   fa.hist = st->d.hist;
+  HIP_TRY(hipEventRecord(g_ctx.evs, g_ctx.stream));
   HIP_TRY(launch_factor(fa, g_ctx.stream));
   HIP_TRY(hipMemsetAsync(st->d.s_hist, 0, (n + 1) * sizeof(unsigned long long), g_ctx.stream));
 
@@ -595,6 +597,9 @@ int run_locked(const mcr_query* q, mcr_result* r) {
   r->near_threshold = (int64_t)acc.near;
   r->kernel = st.plan.kernel;
   r->ms_kernel = ms;
+  float ms_dev = 0.0f;
+  HIP_TRY(hipEventElapsedTime(&ms_dev, g_ctx.evs, g_ctx.ev1));
+  r->ms_device = ms_dev;
   r->var_passes = (q->flags & MCR_F_VAR) ? 1 : 0;
 
   if (q->flags & MCR_F_SHIST) {
@@ -664,12 +669,9 @@ BatchKernel class_kernel(int n) {
   return n <= kSmallMaxN ? batch_small_kernel(n, T) : batch_big_kernel(T);
 }
 
-int class_grid(BatchKernel k) {
-  static std::mutex mu;
-  static std::map<BatchKernel, int> grids;
-  std::lock_guard<std::mutex> lk(mu);
-  auto it = grids.find(k);
-  if (it != grids.end()) return it->second;
+int class_block(int n) { return n <= kSmallMaxN ? 256 : 64 * big_waves((n + 1 + 31) / 32); }
+
+int class_grid(BatchKernel k, int block) {
   int cus = g_ctx.num_cus;
   if (cus <= 0) {
     int dev = 0;
@@ -677,13 +679,12 @@ int class_grid(BatchKernel k) {
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
       cus = 1;
   }
-  const int g = batch_kernel_grid(k, cus);
-  grids[k] = g;
-  return g;
+  return resident_blocks(reinterpret_cast<const void*>(k), block) * cus;
 }
 
 void order_by_class(std::vector<WorkItem>* items, const std::vector<BatchKernel>& qk,
-                    const std::vector<double>& qcost, bool lpt, std::vector<ClassRun>* runs) {
+                    const std::vector<int>& qb, const std::vector<double>& qcost, bool lpt,
+                    std::vector<ClassRun>* runs) {
   // classes by descending total cost, then items longest first (or as given)
   std::map<BatchKernel, double> ccost;
   for (const WorkItem& w : *items) ccost[qk[w.query]] += qcost[w.query] * w.ngroups;
@@ -695,6 +696,8 @@ void order_by_class(std::vector<WorkItem>* items, const std::vector<BatchKernel>
             });
   std::map<BatchKernel, int> rank;
   for (size_t i = 0; i < order.size(); ++i) rank[order[i].second] = (int)i;
+  std::map<BatchKernel, int> block;
+  for (const WorkItem& w : *items) block[qk[w.query]] = qb[w.query];
   std::stable_sort(items->begin(), items->end(), [&](const WorkItem& x, const WorkItem& y) {
     const int rx = rank[qk[x.query]], ry = rank[qk[y.query]];
     if (rx != ry) return rx < ry;
@@ -703,7 +706,8 @@ void order_by_class(std::vector<WorkItem>* items, const std::vector<BatchKernel>
   runs->clear();
   for (size_t i = 0; i < items->size(); ++i) {
     const BatchKernel k = qk[(*items)[i].query];
-    if (runs->empty() || runs->back().k != k) runs->push_back({k, (uint32_t)i, 0u, class_grid(k)});
+    if (runs->empty() || runs->back().k != k)
+      runs->push_back({k, (uint32_t)i, 0u, class_grid(k, block[k]), block[k]});
     runs->back().count++;
   }
 }
@@ -719,7 +723,7 @@ int launch_runs(const std::vector<ClassRun>& runs, const PathArgs* qs, const Wor
     b.nitems = runs[c].count;
     b.next = next + c;
     HIP_TRY(launch_batch_kernel(runs[c].k, b, grid_override > 0 ? grid_override : runs[c].grid,
-                                stream));
+                                runs[c].block, stream));
   }
   return MCR_OK;
 }
@@ -805,17 +809,19 @@ int plan_batch(const mcr_query* qs, int nq, int chunk_paths, int order, BatchPla
   // work items: per class, runs of cg groups, with at least four items per
   // resident workgroup (small classes: per wave) so the class's launch balances
   std::vector<BatchKernel> qk(nq);
+  std::vector<int> qb(nq);
   std::vector<double> qcost(nq);
   std::map<BatchKernel, uint64_t> class_groups;
   for (int i = 0; i < nq; ++i) {
     qk[i] = class_kernel(qs[i].n);
+    qb[i] = class_block(qs[i].n);
     qcost[i] = group_cost(bp->T[i], bp->S[i]);
     class_groups[qk[i]] += ((uint64_t)qs[i].paths + 63) / 64;
   }
   const uint64_t cg = (uint64_t)std::max(1, chunk_paths / 64);
   bp->items.clear();
   for (int i = 0; i < nq; ++i) {
-    const uint64_t slots = (uint64_t)class_grid(qk[i]) * (qs[i].n <= kSmallMaxN ? 4 : 1);
+    const uint64_t slots = (uint64_t)class_grid(qk[i], qb[i]) * (qs[i].n <= kSmallMaxN ? 4 : 1);
     const uint64_t cgi = std::max<uint64_t>(1, std::min(cg, class_groups[qk[i]] / (4 * slots)));
     const uint64_t G = ((uint64_t)qs[i].paths + 63) / 64;
     for (uint64_t f = 0; f < G; f += cgi) {
@@ -826,7 +832,7 @@ int plan_batch(const mcr_query* qs, int nq, int chunk_paths, int order, BatchPla
       bp->items.push_back(w);
     }
   }
-  order_by_class(&bp->items, qk, qcost, order == MCR_ORDER_LPT, &bp->runs);
+  order_by_class(&bp->items, qk, qb, qcost, order == MCR_ORDER_LPT, &bp->runs);
   // layout: [inputs | outputs | scratch]
   Carver c{nullptr};
   bp->offA.resize(nq);
@@ -1135,6 +1141,7 @@ int mcr_shutdown(void) {
   if (g_ctx.host) HIP_TRY(hipHostFree(g_ctx.host));
   HIP_TRY(hipEventDestroy(g_ctx.ev0));
   HIP_TRY(hipEventDestroy(g_ctx.ev1));
+  HIP_TRY(hipEventDestroy(g_ctx.evs));
   HIP_TRY(hipStreamDestroy(g_ctx.stream));
   g_ctx = Context();
   return MCR_OK;

@@ -19,6 +19,9 @@
 // Every path kernel reduces the exceedance count, tail counts and fixed-point
 // tail sums per lane, then per wave, with one atomic per wave per counter; no
 // normal ever leaves the chip.
+#include <map>
+#include <mutex>
+
 #include "mcr_kern_common.h"
 
 namespace mcr {
@@ -580,6 +583,22 @@ hipError_t launch_normals(uint64_t seed, uint64_t path0, uint64_t npaths, int n,
 // ---------------------------------------------------------------------------
 static uint64_t min_u64(uint64_t x, uint64_t y) { return x < y ? x : y; }
 
+int resident_blocks(const void* kernel, int block) {
+  static std::mutex mu;
+  static std::map<std::pair<const void*, int>, int> cache;
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = cache.find({kernel, block});
+  if (it != cache.end()) return it->second;
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0) != hipSuccess ||
+      per_cu < 1) {
+    (void)hipGetLastError();
+    per_cu = 1;
+  }
+  cache[{kernel, block}] = per_cu;
+  return per_cu;
+}
+
 hipError_t launch_paths(int kernel, const PathArgs& a, int num_cus, hipStream_t stream) {
   if (a.npaths == 0) return hipSuccess;
   if (kernel == 1) {
@@ -612,10 +631,10 @@ hipError_t launch_paths(int kernel, const PathArgs& a, int num_cus, hipStream_t 
   if (kernel == 4) {
     PathKernel k = mfma_big_kernel(a.T);
     if (!k || a.n > kMaxN) return hipErrorInvalidValue;
-    // 4-5 row tiles per wave need ~220-255 VGPRs: one or two workgroups per CU
-    const uint64_t per_cu = (a.T + 3) / 4 >= 5 ? 1 : 2;
+    const int block = 64 * big_waves(a.T);
+    const uint64_t per_cu = (uint64_t)resident_blocks((const void*)k, block);
     const uint64_t blocks = min_u64((a.npaths + 63) / 64, (uint64_t)num_cus * per_cu);
-    hipLaunchKernelGGL(k, dim3((unsigned)blocks), dim3(256), 0, stream, a);
+    hipLaunchKernelGGL(k, dim3((unsigned)blocks), dim3(block), 0, stream, a);
     return hipGetLastError();
   }
   if (kernel == 3) {

@@ -50,6 +50,7 @@ struct Context {
   int num_cus = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t evs = nullptr;  // start of a query's device work (after its uploads)
   void* arena = nullptr;
   size_t arena_bytes = 0;
   void* host = nullptr;  // pinned staging
@@ -81,11 +82,14 @@ struct ClassRun {
   BatchKernel k;
   uint32_t off, count;  // item range
   int grid;             // resident workgroups of k
+  int block;            // threads per workgroup of k
 };
 BatchKernel class_kernel(int n);
-int class_grid(BatchKernel k);
+int class_block(int n);
+int class_grid(BatchKernel k, int block);
 void order_by_class(std::vector<WorkItem>* items, const std::vector<BatchKernel>& qk,
-                    const std::vector<double>& qcost, bool lpt, std::vector<ClassRun>* runs);
+                    const std::vector<int>& qb, const std::vector<double>& qcost, bool lpt,
+                    std::vector<ClassRun>* runs);
 int launch_runs(const std::vector<ClassRun>& runs, const PathArgs* qs, const WorkItem* items,
                 unsigned* next, int grid_override, hipStream_t stream);
 void fill_statistics(int64_t M, const DevInfo& info, const DevAccum& acc, mcr_result* r);

@@ -11,7 +11,7 @@ constexpr int kMaxN = 608;      // tile kernel: 64 paths x n normals in LDS (152
 constexpr int kRegMaxN = 64;    // register kernel: one row accumulator per asset
 constexpr int kMfmaMaxT = 4;    // MFMA kernel: 32-row tiles (assets + loss row)
 constexpr int kMfmaNB = 32;     // MFMA kernel: Philox blocks per path (n <= 127)
-constexpr int kBatchTWMax = (kMaxN + 1 + 127) / 128;  // LDS MFMA kernel: tile slots per wave
+
 constexpr int kTilePaths = 64;  // paths per workgroup in the tile kernel
 constexpr int kTileRows = 8;    // rows per register block in the tile kernel
 constexpr unsigned kGatherCap = 1u << 18;  // loss values kept per VaR window
@@ -133,11 +133,13 @@ struct BatchArgs {
 // MFMA code per wave (class = Philox blocks NB, tiles T); larger n runs the
 // LDS MFMA code per workgroup (class = TW = ceil(T / 4) tile slots per wave).
 typedef void (*BatchKernel)(BatchArgs);
-BatchKernel batch_small_kernel(int n, int T);  // n <= 127
-BatchKernel batch_big_kernel(int T);           // T <= 4 * kBatchTWMax
-int batch_kernel_grid(BatchKernel k, int num_cus);  // resident workgroups at 256 threads
+BatchKernel batch_small_kernel(int n, int T);  // n <= 127, 256 threads
+BatchKernel batch_big_kernel(int T);           // T <= 24, 64 * big_waves(T) threads
+int big_waves(int T);                          // waves per workgroup of the LDS MFMA kernel
+int resident_blocks(const void* kernel, int block);  // workgroups per CU (occupancy)
 hipError_t launch_batch_clear(unsigned* next, int count, hipStream_t stream);
-hipError_t launch_batch_kernel(BatchKernel k, const BatchArgs& b, int grid, hipStream_t stream);
+hipError_t launch_batch_kernel(BatchKernel k, const BatchArgs& b, int grid, int block,
+                               hipStream_t stream);
 hipError_t launch_paths(int kernel, const PathArgs& a, int num_cus, hipStream_t stream);
 hipError_t launch_normals(uint64_t seed, uint64_t path0, uint64_t npaths, int n, float* out,
                           hipStream_t stream);

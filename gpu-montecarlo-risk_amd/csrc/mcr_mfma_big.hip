@@ -6,24 +6,32 @@
 // with the same augmented rows as the small MFMA kernel (assets, +inf pads,
 // loss row 32T-1 = -v, initial -w^T mu) and the same bf16 hi/lo split.
 //
-// A workgroup of four waves owns 64 paths.  The 64 x 32T accumulator block is
-// spread over the waves' VGPRs: row tile t belongs to wave (t/4 even ? t%4 :
-// 3-t%4) (snake order, since tile t needs min(2t+2, S) steps), at most TW
-// tiles per wave.  Per 16-wide K step every lane of the workgroup generates one
-// Philox block (4 normals of one path) and writes its bf16 hi/lo parts straight
-// into a double-buffered LDS tile already in B-operand order, so step s+1 is
-// generated while step s is multiplied; factor fragments stream from L2.
+// A workgroup of NW waves (4, or 8 for n >= 512) owns 64 paths.  The 64 x 32T
+// accumulator block is spread over the waves' VGPRs: row tile t belongs to
+// wave (t/NW even ? t%NW : NW-1-t%NW) (snake order, since tile t needs
+// min(2t+2, S) steps), at most TW tiles per wave; the shape keeps at least two
+// waves per SIMD so one's MFMAs overlap another's normal generation.  Per
+// phase of NW/4 16-wide K steps every lane of the workgroup generates one
+// Philox block (4 normals of one path) and writes its bf16 hi/lo parts
+// straight into a double-buffered LDS tile already in B-operand order, so the
+// next phase is generated while this one is multiplied; factor fragments
+// stream from L2.
 #include "mcr_kern_common.h"
 
 namespace mcr {
 
-// LDS operand tiles: [buffer][path tile][hi, lo][operand lane][8 bf16]
+// LDS operand tiles: [buffer][step of the phase][path tile][hi, lo][operand
+// lane][8 bf16].  A phase is NW/4 consecutive 16-wide K steps, generated
+// by the whole workgroup (one Philox block per lane) between two barriers.
+template <int NW>
 struct ZTile {
-  uint4 v[2][2][2][64];
+  uint4 v[2][NW / 4][2][2][64];
 };
 
-// Step s, this lane's block: path rp of the group, normals 16 s + 4 ru .. +3.
-__device__ __forceinline__ void gen_step(ZTile* zt, int buf, int s, int nb, Key2 key,
+// Step s of the group (its normals 16 s .. 16 s + 15 of path rp), this lane's
+// Philox block ru (normals 16 s + 4 ru .. + 3), into phase slot sp of buffer buf.
+template <int NW>
+__device__ __forceinline__ void gen_step(ZTile<NW>* zt, int buf, int sp, int s, int nb, Key2 key,
                                          uint64_t path, int rp, int ru) {
   const int b = 4 * s + ru;
   float z[4];
@@ -37,26 +45,32 @@ __device__ __forceinline__ void gen_step(ZTile* zt, int buf, int s, int nb, Key2
   const uint32_t l1 = pack_bf16x2(z[2] - as_f32(h1 << 16), z[3] - as_f32(h1 & 0xffff0000u));
   const int pt = rp >> 5;
   const int oplane = (rp & 31) + 32 * (ru >> 1);
-  uint2* hi = reinterpret_cast<uint2*>(&zt->v[buf][pt][0][oplane]) + (ru & 1);
-  uint2* lo = reinterpret_cast<uint2*>(&zt->v[buf][pt][1][oplane]) + (ru & 1);
+  uint2* hi = reinterpret_cast<uint2*>(&zt->v[buf][sp][pt][0][oplane]) + (ru & 1);
+  uint2* lo = reinterpret_cast<uint2*>(&zt->v[buf][sp][pt][1][oplane]) + (ru & 1);
   *hi = make_uint2(h0, h1);
   *lo = make_uint2(l0, l1);
 }
 
+template <int NW>
 struct BigShared {
-  ZTile zt;
-  uint32_t sS[4][64];
+  ZTile<NW> zt;
+  uint32_t sS[NW][64];
   float sLoss[64];
 };
 
 // Groups g0, g0 + gstep, ... < g1 of 64 paths of query a, accumulated into t.
-template <int TW>
+// NW waves share each group: row tile t belongs to the wave dealt by a snake
+// over the waves (tile t needs min(2t+2, S) steps, so the deal balances the
+// triangle), at most TW tiles per wave.  A phase is P = NW/4 K steps.
+template <int TW, int NW>
 __device__ __forceinline__ void mfma_lds_groups(const PathArgs& a, uint64_t g0, uint64_t g1,
-                                                uint64_t gstep, BigShared& sh, LaneTally& t,
+                                                uint64_t gstep, BigShared<NW>& sh, LaneTally& t,
                                                 unsigned* sh_hist) {
+  constexpr int P = NW / 4;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 5;
   const int n = a.n, T = a.T, S = a.S;
   const int nb = (n + 3) >> 2;
+  const int nph = (S + P - 1) / P;
   const Key2 key = key_of(a.seed);
   const uint4* __restrict__ lf = a.lfrag + lane * 2;
   const float4* __restrict__ rinit = (const float4*)a.rowinit;
@@ -64,51 +78,58 @@ __device__ __forceinline__ void mfma_lds_groups(const PathArgs& a, uint64_t g0, 
   int tile[TW], steps[TW];
 #pragma unroll
   for (int i = 0; i < TW; ++i) {
-    tile[i] = 4 * i + ((i & 1) ? 3 - wave : wave);
+    tile[i] = NW * i + ((i & 1) ? NW - 1 - wave : wave);
     steps[i] = tile[i] >= T ? 0 : (tile[i] == T - 1 ? S : (2 * tile[i] + 2 < S ? 2 * tile[i] + 2 : S));
   }
-  const int loss_slot = (T - 1) / 4;  // i of tile T-1 (its owner wave)
-  const int loss_wave = (loss_slot & 1) ? 3 - (T - 1) % 4 : (T - 1) % 4;
+  const int loss_slot = (T - 1) / NW;  // i of tile T-1 (its owner wave)
+  const int loss_wave = (loss_slot & 1) ? NW - 1 - (T - 1) % NW : (T - 1) % NW;
 
-  const int rp = threadIdx.x & 63, ru = threadIdx.x >> 6;
+  // generator role of this thread: path rp, block ru of step sp of a phase
+  const int rp = threadIdx.x & 63, ru = (threadIdx.x >> 6) & 3, sp = threadIdx.x >> 8;
   for (uint64_t grp = g0; grp < g1; grp += gstep) {
     const uint64_t rpath = a.path0 + grp * 64 + rp;
-    gen_step(&sh.zt, 0, 0, nb, key, rpath, rp, ru);
+    if (sp < S) gen_step<NW>(&sh.zt, 0, sp, sp, nb, key, rpath, rp, ru);
     __syncthreads();
     f32x16 accA[TW], accB[TW];
-    for (int s = 0; s < S; ++s) {
-      if (s + 1 < S) gen_step(&sh.zt, (s + 1) & 1, s + 1, nb, key, rpath, rp, ru);
-      const int cb = s & 1;
-      const bf16x8 ah = __builtin_bit_cast(bf16x8, sh.zt.v[cb][0][0][lane]);
-      const bf16x8 al = __builtin_bit_cast(bf16x8, sh.zt.v[cb][0][1][lane]);
-      const bf16x8 bh = __builtin_bit_cast(bf16x8, sh.zt.v[cb][1][0][lane]);
-      const bf16x8 bl = __builtin_bit_cast(bf16x8, sh.zt.v[cb][1][1][lane]);
+    for (int ph = 0; ph < nph; ++ph) {
+      const int sn = (ph + 1) * P + sp;
+      if (ph + 1 < nph && sn < S) gen_step<NW>(&sh.zt, (ph + 1) & 1, sp, sn, nb, key, rpath, rp, ru);
+      const int cb = ph & 1;
 #pragma unroll
-      for (int i = 0; i < TW; ++i) {
-        if (s < steps[i]) {
-          const size_t e = (size_t)(tile[i] * S + s) * 128;
-          const bf16x8 Lhi = __builtin_bit_cast(bf16x8, lf[e]);
-          const bf16x8 Llo = __builtin_bit_cast(bf16x8, lf[e + 1]);
-          if (s == 0) {
-            f32x16 init;
+      for (int q = 0; q < P; ++q) {
+        const int s = ph * P + q;
+        if (s >= S) break;
+        const bf16x8 ah = __builtin_bit_cast(bf16x8, sh.zt.v[cb][q][0][0][lane]);
+        const bf16x8 al = __builtin_bit_cast(bf16x8, sh.zt.v[cb][q][0][1][lane]);
+        const bf16x8 bh = __builtin_bit_cast(bf16x8, sh.zt.v[cb][q][1][0][lane]);
+        const bf16x8 bl = __builtin_bit_cast(bf16x8, sh.zt.v[cb][q][1][1][lane]);
 #pragma unroll
-            for (int r4 = 0; r4 < 4; ++r4) {
-              const float4 v = rinit[(32 * tile[i] + 8 * r4 + 4 * g) >> 2];
-              init[4 * r4 + 0] = v.x;
-              init[4 * r4 + 1] = v.y;
-              init[4 * r4 + 2] = v.z;
-              init[4 * r4 + 3] = v.w;
+        for (int i = 0; i < TW; ++i) {
+          if (s < steps[i]) {
+            const size_t e = (size_t)(tile[i] * S + s) * 128;
+            const bf16x8 Lhi = __builtin_bit_cast(bf16x8, lf[e]);
+            const bf16x8 Llo = __builtin_bit_cast(bf16x8, lf[e + 1]);
+            if (s == 0) {
+              f32x16 init;
+#pragma unroll
+              for (int r4 = 0; r4 < 4; ++r4) {
+                const float4 v = rinit[(32 * tile[i] + 8 * r4 + 4 * g) >> 2];
+                init[4 * r4 + 0] = v.x;
+                init[4 * r4 + 1] = v.y;
+                init[4 * r4 + 2] = v.z;
+                init[4 * r4 + 3] = v.w;
+              }
+              accA[i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Llo, ah, init, 0, 0, 0);
+              accB[i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Llo, bh, init, 0, 0, 0);
+            } else {
+              accA[i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Llo, ah, accA[i], 0, 0, 0);
+              accB[i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Llo, bh, accB[i], 0, 0, 0);
             }
-            accA[i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Llo, ah, init, 0, 0, 0);
-            accB[i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Llo, bh, init, 0, 0, 0);
-          } else {
-            accA[i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Llo, ah, accA[i], 0, 0, 0);
-            accB[i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Llo, bh, accB[i], 0, 0, 0);
+            accA[i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Lhi, al, accA[i], 0, 0, 0);
+            accB[i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Lhi, bl, accB[i], 0, 0, 0);
+            accA[i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Lhi, ah, accA[i], 0, 0, 0);
+            accB[i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Lhi, bh, accB[i], 0, 0, 0);
           }
-          accA[i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Lhi, al, accA[i], 0, 0, 0);
-          accB[i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Lhi, bl, accB[i], 0, 0, 0);
-          accA[i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Lhi, ah, accA[i], 0, 0, 0);
-          accB[i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Lhi, bh, accB[i], 0, 0, 0);
         }
       }
       __syncthreads();
@@ -144,16 +165,18 @@ __device__ __forceinline__ void mfma_lds_groups(const PathArgs& a, uint64_t g0, 
     __syncthreads();
     const uint64_t q = grp * 64 + lane;
     if (wave == 0 && q < a.npaths) {
-      const int Sp = (int)(sh.sS[0][lane] + sh.sS[1][lane] + sh.sS[2][lane] + sh.sS[3][lane]);
+      int Sp = 0;
+#pragma unroll
+      for (int w = 0; w < NW; ++w) Sp += (int)sh.sS[w][lane];
       tally_path(a, t, q, Sp, sh.sLoss[lane], false, sh_hist);
     }
     __syncthreads();
   }
 }
 
-template <int TW>
-__global__ __launch_bounds__(256) void k_mfma_big(PathArgs a) {
-  __shared__ BigShared sh;
+template <int TW, int NW>
+__global__ __launch_bounds__(64 * NW) void k_mfma_big(PathArgs a) {
+  __shared__ BigShared<NW> sh;
   __shared__ unsigned sh_hist[kMaxN + 1];
   const bool want_hist = a.s_hist != nullptr;
   if (want_hist) {
@@ -161,8 +184,8 @@ __global__ __launch_bounds__(256) void k_mfma_big(PathArgs a) {
   }
   LaneTally t;
   zero_tally(t);
-  mfma_lds_groups<TW>(a, blockIdx.x, (a.npaths + 63) / 64, gridDim.x, sh, t,
-                      want_hist ? sh_hist : nullptr);
+  mfma_lds_groups<TW, NW>(a, blockIdx.x, (a.npaths + 63) / 64, gridDim.x, sh, t,
+                          want_hist ? sh_hist : nullptr);
   flush_tally(a, t);
   if (want_hist) {
     __syncthreads();
@@ -176,9 +199,9 @@ __global__ __launch_bounds__(256) void k_mfma_big(PathArgs a) {
 // the list is exhausted; the exit test is uniform, so every wave of every
 // workgroup leaves the loop.  Tallies are flushed into the item's query after
 // each item.
-template <int TW>
-__global__ __launch_bounds__(256) void k_batch_paths(BatchArgs b) {
-  __shared__ BigShared sh;
+template <int TW, int NW>
+__global__ __launch_bounds__(64 * NW) void k_batch_paths(BatchArgs b) {
+  __shared__ BigShared<NW> sh;
   __shared__ unsigned s_item;
   for (;;) {
     if (threadIdx.x == 0) s_item = atomicAdd(b.next, 1u);
@@ -192,7 +215,7 @@ __global__ __launch_bounds__(256) void k_batch_paths(BatchArgs b) {
     const PathArgs a = load_const(b.qs + w.query);
     LaneTally t;
     zero_tally(t);
-    mfma_lds_groups<TW>(a, w.first_group, w.first_group + w.ngroups, 1, sh, t, nullptr);
+    mfma_lds_groups<TW, NW>(a, w.first_group, w.first_group + w.ngroups, 1, sh, t, nullptr);
     flush_tally(a, t);
   }
 }
@@ -206,42 +229,41 @@ hipError_t launch_batch_clear(unsigned* next, int count, hipStream_t stream) {
   return hipGetLastError();
 }
 
-hipError_t launch_batch_kernel(BatchKernel k, const BatchArgs& b, int grid, hipStream_t stream) {
-  hipLaunchKernelGGL(k, dim3(grid), dim3(256), 0, stream, b);
+hipError_t launch_batch_kernel(BatchKernel k, const BatchArgs& b, int grid, int block,
+                               hipStream_t stream) {
+  hipLaunchKernelGGL(k, dim3(grid), dim3(block), 0, stream, b);
   return hipGetLastError();
 }
 
-int batch_kernel_grid(BatchKernel k, int num_cus) {
-  int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(k), 256,
-                                                   0) != hipSuccess ||
-      per_cu < 1) {
-    (void)hipGetLastError();
-    per_cu = 1;
-  }
-  return per_cu * num_cus;
-}
+// Shape of the LDS MFMA kernel for T row tiles: four waves (several
+// workgroups per CU, so one's barrier waits are covered by another's work)
+// while 4 tiles per wave fit in half the register file; eight waves with 3
+// tiles each beyond that, where four waves would need 5 tiles and run one
+// wave per SIMD.
+int big_waves(int T) { return T <= 16 ? 4 : 8; }
 
 BatchKernel batch_big_kernel(int T) {
-  switch ((T + 3) / 4) {
-    case 1: return k_batch_paths<1>;
-    case 2: return k_batch_paths<2>;
-    case 3: return k_batch_paths<3>;
-    case 4: return k_batch_paths<4>;
-    case 5: return k_batch_paths<5>;
-    default: return nullptr;
+  if (T <= 16) {
+    switch ((T + 3) / 4) {
+      case 1: return k_batch_paths<1, 4>;
+      case 2: return k_batch_paths<2, 4>;
+      case 3: return k_batch_paths<3, 4>;
+      case 4: return k_batch_paths<4, 4>;
+    }
   }
+  return T <= 24 ? k_batch_paths<3, 8> : nullptr;
 }
 
 PathKernel mfma_big_kernel(int T) {
-  switch ((T + 3) / 4) {
-    case 1: return k_mfma_big<1>;
-    case 2: return k_mfma_big<2>;
-    case 3: return k_mfma_big<3>;
-    case 4: return k_mfma_big<4>;
-    case 5: return k_mfma_big<5>;
-    default: return nullptr;
+  if (T <= 16) {
+    switch ((T + 3) / 4) {
+      case 1: return k_mfma_big<1, 4>;
+      case 2: return k_mfma_big<2, 4>;
+      case 3: return k_mfma_big<3, 4>;
+      case 4: return k_mfma_big<4, 4>;
+    }
   }
+  return T <= 24 ? k_mfma_big<3, 8> : nullptr;
 }
 
 }  // namespace mcr

@@ -597,12 +597,14 @@ int mcr_server::plan_round(uint64_t round, RoundRec* rr, std::vector<WorkItem>* 
   }
   for (auto& x : it) items->push_back(x.second);
   std::vector<BatchKernel> qk(max_active, nullptr);
+  std::vector<int> qb(max_active, 0);
   std::vector<double> qcost(max_active, 0.0);
   for (auto& gr : grant) {
     qk[gr.first->slot] = class_kernel(gr.first->q.n);
+    qb[gr.first->slot] = class_block(gr.first->q.n);
     qcost[gr.first->slot] = group_cost(gr.first->T, gr.first->S);
   }
-  order_by_class(items, qk, qcost, true, &rr->runs);
+  order_by_class(items, qk, qb, qcost, true, &rr->runs);
   if (rr->runs.size() > (size_t)kMaxRuns) {
     set_error("round needs %zu kernel classes (limit %d)", rr->runs.size(), kMaxRuns);
     return MCR_E_DEVICE;

@@ -98,6 +98,9 @@ typedef struct mcr_result {
   int kernel;             /* MCR_KERNEL_* that ran */
   double ms_kernel;       /* path kernel time (HIP events) */
   double ms_total;        /* host wall time of the call */
+  double ms_device;       /* device time from the factor to the end of the path
+                             kernels (HIP events; inputs already uploaded, no
+                             host<->device copies); first VaR pass only */
 } mcr_result;
 
 typedef struct mcr_factor_info {
