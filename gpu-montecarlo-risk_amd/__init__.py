@@ -27,6 +27,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_lib", "libmcr_hip.so")
 HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "mcr_hip.h")
+HEADER_PATHS = [HEADER_PATH, os.path.join(os.path.dirname(_HERE), "include", "mcr_io.h")]
 
 # constants from include/mcr_hip.h
 MCR_OK = 0
@@ -154,8 +155,9 @@ def lib() -> ctypes.CDLL:
 
 
 def declared_functions() -> list[str]:
-    """Function names declared in include/mcr_hip.h (the exported C ABI)."""
-    text = open(HEADER_PATH).read()
+    """Function names declared in include/mcr_hip.h and include/mcr_io.h (the
+    exported C ABI)."""
+    text = "\n".join(open(p).read() for p in HEADER_PATHS)
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return re.findall(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_]*\s*\*?\s*([a-z_][a-z0-9_]*)\s*\(",
                       text, flags=re.M)
@@ -511,3 +513,160 @@ def make_params(mu, Sigma, x: float, k: int, M: int, seed: int, *, mu_stride: in
     p = MonteCarloParams(float(x), n, int(k), ctypes.pointer(gv), ctypes.pointer(gm), int(M),
                          int(seed))
     return p, (mbuf, sbuf, gv, gm)
+
+
+# ---------------------------------------------------------------- data I/O ---
+class _Index(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char_p), ("file", ctypes.c_char_p)]
+
+
+class _Config(ctypes.Structure):
+    _fields_ = [("start", ctypes.c_char_p), ("end", ctypes.c_char_p), ("x", ctypes.c_double),
+                ("N", ctypes.c_int), ("M", ctypes.c_int), ("k", ctypes.c_int),
+                ("random_seed", ctypes.c_ulong), ("train_ratio", ctypes.c_double),
+                ("indices", ctypes.POINTER(_Index)), ("num_indices", ctypes.c_int),
+                ("models", ctypes.POINTER(ctypes.c_char_p)), ("num_models", ctypes.c_int),
+                ("comment", ctypes.c_char_p)]
+
+
+class CsvRow(ctypes.Structure):
+    _fields_ = [("iteration_id", ctypes.c_int), ("timestamp", ctypes.c_long),
+                ("execution_time_ms", ctypes.c_long), ("comment", ctypes.c_char_p),
+                ("start_date", ctypes.c_char_p), ("end_date", ctypes.c_char_p),
+                ("train_ratio", ctypes.c_double), ("M", ctypes.c_int), ("k", ctypes.c_int),
+                ("x", ctypes.c_double), ("model_name", ctypes.c_char_p),
+                ("seed", ctypes.c_ulong), ("nodes", ctypes.c_int), ("threads", ctypes.c_int),
+                ("processes", ctypes.c_int), ("indices", ctypes.POINTER(_Index)),
+                ("num_indices", ctypes.c_int), ("actual_freq", ctypes.c_double),
+                ("P_hat", ctypes.c_double), ("count", ctypes.c_int),
+                ("std_error", ctypes.c_double), ("ci_lower", ctypes.c_double),
+                ("ci_upper", ctypes.c_double)]
+
+
+RUNNER_PATH = os.path.join(_HERE, "_lib", "mcr_runner")
+
+
+def _io_lib():
+    L = lib()
+    if not getattr(L, "_io_sigs", False):
+        dpp = ctypes.POINTER(ctypes.POINTER(ctypes.c_double))
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.mcr_read_mu_bin.argtypes = [ctypes.c_char_p, dpp, ip]
+        L.mcr_read_sigma_bin.argtypes = [ctypes.c_char_p, dpp, ip, ip]
+        L.mcr_read_params_bin.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double)]
+        L.mcr_write_bins.argtypes = [ctypes.c_char_p, _f64p, ctypes.c_int, _f64p,
+                                     ctypes.c_double]
+        L.mcr_free.argtypes = [ctypes.c_void_p]
+        L.mcr_load_config.argtypes = [ctypes.c_char_p, ctypes.POINTER(_Config)]
+        L.mcr_free_config.argtypes = [ctypes.POINTER(_Config)]
+        L.mcr_csv_header.argtypes = [ctypes.c_char_p, ctypes.c_int]
+        L.mcr_csv_format_row.argtypes = [ctypes.POINTER(CsvRow), ctypes.c_char_p, ctypes.c_int]
+        L.mcr_append_csv.argtypes = [ctypes.c_char_p, ctypes.POINTER(CsvRow)]
+        L._io_sigs = True
+    return L
+
+
+def read_mu_bin(path) -> np.ndarray:
+    """mu.bin (int32 N + N float64), as load_mu_binary reads it."""
+    L = _io_lib()
+    p, n = ctypes.POINTER(ctypes.c_double)(), ctypes.c_int(0)
+    _check(L.mcr_read_mu_bin(os.fsencode(path), ctypes.byref(p), ctypes.byref(n)))
+    out = np.ctypeslib.as_array(p, shape=(n.value,)).copy()
+    L.mcr_free(p)
+    return out
+
+
+def read_sigma_bin(path) -> np.ndarray:
+    """sigma.bin (int32 rows, int32 cols + row-major float64)."""
+    L = _io_lib()
+    p, r, c = ctypes.POINTER(ctypes.c_double)(), ctypes.c_int(0), ctypes.c_int(0)
+    _check(L.mcr_read_sigma_bin(os.fsencode(path), ctypes.byref(p), ctypes.byref(r),
+                                ctypes.byref(c)))
+    out = np.ctypeslib.as_array(p, shape=(r.value, c.value)).copy()
+    L.mcr_free(p)
+    return out
+
+
+def read_params_bin(path) -> float:
+    """params.bin (one float64 in [0, 1])."""
+    f = ctypes.c_double(0.0)
+    _check(_io_lib().mcr_read_params_bin(os.fsencode(path), ctypes.byref(f)))
+    return f.value
+
+
+def write_bins(out_dir, mu, sigma, actual_freq: float) -> None:
+    """out_dir/mu.bin, sigma.bin, params.bin in the reference's format."""
+    mu = np.ascontiguousarray(mu, dtype=np.float64)
+    sigma = np.ascontiguousarray(sigma, dtype=np.float64)
+    n = int(mu.shape[0])
+    if sigma.shape != (n, n):
+        raise ValueError(f"sigma must be {n} x {n}")
+    _check(_io_lib().mcr_write_bins(os.fsencode(out_dir), _ptr(mu, _f64p), n, _ptr(sigma, _f64p),
+                                    float(actual_freq)))
+
+
+def load_config(path) -> dict:
+    """config.yaml as the C loader reads it (mcr_load_config)."""
+    L = _io_lib()
+    c = _Config()
+    _check(L.mcr_load_config(os.fsencode(path), ctypes.byref(c)))
+    dec = lambda b: None if b is None else b.decode()  # noqa: E731
+    out = {"start": dec(c.start), "end": dec(c.end), "x": c.x, "N": c.N, "M": c.M, "k": c.k,
+           "random_seed": c.random_seed, "train_ratio": c.train_ratio,
+           "indices": {c.indices[i].name.decode(): c.indices[i].file.decode()
+                       for i in range(c.num_indices)},
+           "models": [c.models[i].decode() for i in range(c.num_models)],
+           "comment": dec(c.comment)}
+    L.mcr_free_config(ctypes.byref(c))
+    return out
+
+
+def _csv_row(fields: dict):
+    enc = lambda s: None if s is None else str(s).encode()  # noqa: E731
+    idx = list((fields.get("indices") or {}).items())
+    arr = (_Index * max(1, len(idx)))()
+    for i, (nm, fl) in enumerate(idx):
+        arr[i] = _Index(enc(nm), enc(fl))
+    row = CsvRow()
+    for name, _t in CsvRow._fields_:
+        if name in ("indices", "num_indices") or name not in fields:
+            continue
+        v = fields[name]
+        setattr(row, name, enc(v) if name in ("comment", "start_date", "end_date",
+                                               "model_name") else v)
+    row.indices = arr if idx else None
+    row.num_indices = len(idx)
+    return row, arr
+
+
+def csv_header() -> str:
+    buf = ctypes.create_string_buffer(1024)
+    n = _io_lib().mcr_csv_header(buf, len(buf))
+    if n < 0:
+        _check(n)
+    return buf.value.decode()
+
+
+def csv_format_row(**fields) -> str:
+    """The results-CSV line for the given fields (names of mcr_csv_row)."""
+    row, keep = _csv_row(fields)
+    buf = ctypes.create_string_buffer(1 << 16)
+    n = _io_lib().mcr_csv_format_row(ctypes.byref(row), buf, len(buf))
+    if n < 0:
+        _check(n)
+    del keep
+    return buf.value.decode()
+
+
+def append_csv(path, **fields) -> None:
+    row, keep = _csv_row(fields)
+    _check(_io_lib().mcr_append_csv(os.fsencode(path), ctypes.byref(row)))
+    del keep
+
+
+def preprocessing():
+    """The real-data preprocessing module (preprocess.py)."""
+    import importlib
+
+    return importlib.import_module(__name__ + ".preprocess")
+

@@ -12,7 +12,8 @@ import pytest
 
 def test_library_exports_every_declared_function(mcr):
     names = mcr.declared_functions()
-    assert {"mcr_run", "mcr_factor", "get_hip_model", "hip_simulate"} <= set(names)
+    assert {"mcr_run", "mcr_factor", "get_hip_model", "hip_simulate", "mcr_server_submit",
+            "mcr_read_mu_bin", "mcr_load_config", "mcr_append_csv"} <= set(names)
     L = ctypes.CDLL(mcr.LIB_PATH)
     missing = [n for n in names if not hasattr(L, n)]
     assert not missing, missing
