@@ -61,8 +61,11 @@ struct BigShared {
 // Groups g0, g0 + gstep, ... < g1 of 64 paths of query a, accumulated into t.
 // NW waves share each group: row tile t belongs to the wave dealt by a snake
 // over the waves (tile t needs min(2t+2, S) steps, so the deal balances the
-// triangle), at most TW tiles per wave.  A phase is P = NW/4 K steps.
-template <int TW, int NW>
+// triangle), at most TW tiles per wave.  A phase is P = NW/4 K steps.  With
+// PF the next K step's factor fragments are loaded before this step's MFMAs
+// (also across the phase barrier); that costs 8 TW registers, so it is used
+// only where it does not lower the occupancy.
+template <int TW, int NW, bool PF>
 __device__ __forceinline__ void mfma_lds_groups(const PathArgs& a, uint64_t g0, uint64_t g1,
                                                 uint64_t gstep, BigShared<NW>& sh, LaneTally& t,
                                                 unsigned* sh_hist) {
@@ -91,6 +94,17 @@ __device__ __forceinline__ void mfma_lds_groups(const PathArgs& a, uint64_t g0, 
     if (sp < S) gen_step<NW>(&sh.zt, 0, sp, sp, nb, key, rpath, rp, ru);
     __syncthreads();
     f32x16 accA[TW], accB[TW];
+    // factor fragments of the current K step; the next step's are loaded
+    // before this step's MFMAs (also across the phase barrier)
+    uint4 Lh[TW], Ll[TW];
+#pragma unroll
+    for (int i = 0; i < TW; ++i) {
+      if (PF && steps[i] > 0) {
+        const size_t e = (size_t)(tile[i] * S) * 128;
+        Lh[i] = lf[e];
+        Ll[i] = lf[e + 1];
+      }
+    }
     for (int ph = 0; ph < nph; ++ph) {
       const int sn = (ph + 1) * P + sp;
       if (ph + 1 < nph && sn < S) gen_step<NW>(&sh.zt, (ph + 1) & 1, sp, sn, nb, key, rpath, rp, ru);
@@ -99,6 +113,15 @@ __device__ __forceinline__ void mfma_lds_groups(const PathArgs& a, uint64_t g0, 
       for (int q = 0; q < P; ++q) {
         const int s = ph * P + q;
         if (s >= S) break;
+        uint4 Nh[TW], Nl[TW];
+#pragma unroll
+        for (int i = 0; i < TW; ++i) {
+          if (PF && s + 1 < steps[i]) {
+            const size_t e = (size_t)(tile[i] * S + s + 1) * 128;
+            Nh[i] = lf[e];
+            Nl[i] = lf[e + 1];
+          }
+        }
         const bf16x8 ah = __builtin_bit_cast(bf16x8, sh.zt.v[cb][q][0][0][lane]);
         const bf16x8 al = __builtin_bit_cast(bf16x8, sh.zt.v[cb][q][0][1][lane]);
         const bf16x8 bh = __builtin_bit_cast(bf16x8, sh.zt.v[cb][q][1][0][lane]);
@@ -106,9 +129,13 @@ __device__ __forceinline__ void mfma_lds_groups(const PathArgs& a, uint64_t g0, 
 #pragma unroll
         for (int i = 0; i < TW; ++i) {
           if (s < steps[i]) {
-            const size_t e = (size_t)(tile[i] * S + s) * 128;
-            const bf16x8 Lhi = __builtin_bit_cast(bf16x8, lf[e]);
-            const bf16x8 Llo = __builtin_bit_cast(bf16x8, lf[e + 1]);
+            if (!PF) {
+              const size_t e = (size_t)(tile[i] * S + s) * 128;
+              Lh[i] = lf[e];
+              Ll[i] = lf[e + 1];
+            }
+            const bf16x8 Lhi = __builtin_bit_cast(bf16x8, Lh[i]);
+            const bf16x8 Llo = __builtin_bit_cast(bf16x8, Ll[i]);
             if (s == 0) {
               f32x16 init;
 #pragma unroll
@@ -129,6 +156,13 @@ __device__ __forceinline__ void mfma_lds_groups(const PathArgs& a, uint64_t g0, 
             accB[i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Lhi, bl, accB[i], 0, 0, 0);
             accA[i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Lhi, ah, accA[i], 0, 0, 0);
             accB[i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Lhi, bh, accB[i], 0, 0, 0);
+          }
+        }
+#pragma unroll
+        for (int i = 0; i < TW; ++i) {
+          if (PF && s + 1 < steps[i]) {
+            Lh[i] = Nh[i];
+            Ll[i] = Nl[i];
           }
         }
       }
@@ -174,7 +208,7 @@ __device__ __forceinline__ void mfma_lds_groups(const PathArgs& a, uint64_t g0, 
   }
 }
 
-template <int TW, int NW>
+template <int TW, int NW, bool PF>
 __global__ __launch_bounds__(64 * NW) void k_mfma_big(PathArgs a) {
   __shared__ BigShared<NW> sh;
   __shared__ unsigned sh_hist[kMaxN + 1];
@@ -184,7 +218,7 @@ __global__ __launch_bounds__(64 * NW) void k_mfma_big(PathArgs a) {
   }
   LaneTally t;
   zero_tally(t);
-  mfma_lds_groups<TW, NW>(a, blockIdx.x, (a.npaths + 63) / 64, gridDim.x, sh, t,
+  mfma_lds_groups<TW, NW, PF>(a, blockIdx.x, (a.npaths + 63) / 64, gridDim.x, sh, t,
                           want_hist ? sh_hist : nullptr);
   flush_tally(a, t);
   if (want_hist) {
@@ -199,7 +233,7 @@ __global__ __launch_bounds__(64 * NW) void k_mfma_big(PathArgs a) {
 // the list is exhausted; the exit test is uniform, so every wave of every
 // workgroup leaves the loop.  Tallies are flushed into the item's query after
 // each item.
-template <int TW, int NW>
+template <int TW, int NW, bool PF>
 __global__ __launch_bounds__(64 * NW) void k_batch_paths(BatchArgs b) {
   __shared__ BigShared<NW> sh;
   __shared__ unsigned s_item;
@@ -215,7 +249,7 @@ __global__ __launch_bounds__(64 * NW) void k_batch_paths(BatchArgs b) {
     const PathArgs a = load_const(b.qs + w.query);
     LaneTally t;
     zero_tally(t);
-    mfma_lds_groups<TW, NW>(a, w.first_group, w.first_group + w.ngroups, 1, sh, t, nullptr);
+    mfma_lds_groups<TW, NW, PF>(a, w.first_group, w.first_group + w.ngroups, 1, sh, t, nullptr);
     flush_tally(a, t);
   }
 }
@@ -239,31 +273,24 @@ hipError_t launch_batch_kernel(BatchKernel k, const BatchArgs& b, int grid, int 
 // workgroups per CU, so one's barrier waits are covered by another's work)
 // while 4 tiles per wave fit in half the register file; eight waves with 3
 // tiles each beyond that, where four waves would need 5 tiles and run one
-// wave per SIMD.
+// wave per SIMD.  Fragment prefetch where it keeps the occupancy (measured:
+// -16 % at n = 600; +14..60 % where it cost a wave per SIMD).
 int big_waves(int T) { return T <= 16 ? 4 : 8; }
 
 BatchKernel batch_big_kernel(int T) {
-  if (T <= 16) {
-    switch ((T + 3) / 4) {
-      case 1: return k_batch_paths<1, 4>;
-      case 2: return k_batch_paths<2, 4>;
-      case 3: return k_batch_paths<3, 4>;
-      case 4: return k_batch_paths<4, 4>;
-    }
-  }
-  return T <= 24 ? k_batch_paths<3, 8> : nullptr;
+  if (T <= 4) return k_batch_paths<1, 4, true>;
+  if (T <= 8) return k_batch_paths<2, 4, false>;
+  if (T <= 12) return k_batch_paths<3, 4, false>;
+  if (T <= 16) return k_batch_paths<4, 4, false>;
+  return T <= 24 ? k_batch_paths<3, 8, true> : nullptr;
 }
 
 PathKernel mfma_big_kernel(int T) {
-  if (T <= 16) {
-    switch ((T + 3) / 4) {
-      case 1: return k_mfma_big<1, 4>;
-      case 2: return k_mfma_big<2, 4>;
-      case 3: return k_mfma_big<3, 4>;
-      case 4: return k_mfma_big<4, 4>;
-    }
-  }
-  return T <= 24 ? k_mfma_big<3, 8> : nullptr;
+  if (T <= 4) return k_mfma_big<1, 4, true>;
+  if (T <= 8) return k_mfma_big<2, 4, false>;
+  if (T <= 12) return k_mfma_big<3, 4, false>;
+  if (T <= 16) return k_mfma_big<4, 4, false>;
+  return T <= 24 ? k_mfma_big<3, 8, true> : nullptr;
 }
 
 }  // namespace mcr
