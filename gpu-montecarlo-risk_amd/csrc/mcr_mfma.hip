@@ -8,24 +8,27 @@ namespace mcr {
 // MFMA kernel: T tiles of 32 rows (assets, zero pads, loss row 32T-1), S steps
 // of 16 normals, v_mfma_f32_32x32x16_bf16 with A = factor rows (bf16 hi/lo,
 // resident in registers for the whole kernel) and B = normals (bf16 hi/lo).
-// A wave handles 64 paths, lane l generating every normal of path l.  For
-// step s, X = z[16s..16s+7] and Y = z[16s+8..16s+15] of the lane's own path;
-// one v_permlane32_swap(X, Y) turns them into the B operands of the two
-// 32-path tiles ([X_lo | Y_lo] for paths 0-31, [X_hi | Y_hi] for paths
-// 32-63).  D[row][path]: lane l holds path l%32 of a tile and rows
-// 8(r/4) + 4(l/32) + r%4; two more swaps bring each path's crash count and
-// loss back to lane l.
+// A wave handles 64 paths as two 32-path tiles; for a tile, the two lanes l
+// and l + 32 share path l % 32, generating its normals 16s..16s+7 and
+// 16s+8..16s+15 of each step s: their B-operand slots.  D[row][path]: lane l
+// holds rows 8(r/4) + 4(l/32) + r%4 of its path; one v_permlane32_swap sums
+// the two halves' crash counts, and the upper lane, which holds the loss row,
+// tallies the path.
 // ---------------------------------------------------------------------------
 // NB = Philox blocks per path (n <= 4 NB), T = row tiles; both compile-time
 // so every block, step and tile bound below is a constant.
-// Groups g0, g0 + gstep, ... < g1 of 64 paths of query a, one group per wave
-// at a time, accumulated into the lane tally t (no workgroup barriers).
+// Groups g0, g0 + gstep, ... < g1 of 64 paths of query a, accumulated into
+// the lane tally t (no workgroup barriers).  A wave runs a group as two
+// 32-path tiles.  For tile path p = lane & 31, lane l supplies normals
+// 16s + 8h .. 16s + 8h + 7 of step s (h = l >> 5, the half-wave): exactly its
+// slot of the B operand (lane l: column l % 32, k = 8 (l / 32) + e), so the
+// normals need no lane exchange and the half-wave pair shares one path.
 template <int NB, int T>
 __device__ __forceinline__ void mfma_wave_groups(const PathArgs& a, uint64_t g0, uint64_t g1,
                                                  uint64_t gstep, LaneTally& t, unsigned* sh_hist) {
   constexpr int S = (NB + 3) / 4;  // 16-wide K steps
   const int lane = threadIdx.x & 63;
-  const int g = lane >> 5;
+  const int h = lane >> 5;
   const Key2 key = key_of(a.seed);
   // Factor fragments are streamed per step from L1/L2 (16 B per lane per
   // tile and part) instead of being held for the whole kernel.
@@ -35,86 +38,74 @@ __device__ __forceinline__ void mfma_wave_groups(const PathArgs& a, uint64_t g0,
   const float4* __restrict__ rinit = (const float4*)a.rowinit;
 
   for (uint64_t grp = g0; grp < g1; grp += gstep) {
-    const uint64_t q = grp * 64 + lane;
-    const uint64_t path = a.path0 + q;
-    f32x16 accA[T], accB[T];
+#pragma unroll 1
+    for (int sub = 0; sub < 2; ++sub) {
+      const uint64_t q = grp * 64 + 32 * sub + (lane & 31);
+      const uint64_t path = a.path0 + q;
+      f32x16 acc[T];
 #pragma unroll
-    for (int s = 0; s < S; ++s) {
-      bf16x8 Lhi[T], Llo[T];
+      for (int s = 0; s < S; ++s) {
+        bf16x8 Lhi[T], Llo[T];
 #pragma unroll
-      for (int tt = 0; tt < T; ++tt) {
-        const int steps = (tt == T - 1) ? S : (2 * tt + 2 < S ? 2 * tt + 2 : S);
-        if (s < steps) {
-          const size_t e = (size_t)(tt * S + s) * 128;
-          Lhi[tt] = __builtin_bit_cast(bf16x8, lf[e]);
-          Llo[tt] = __builtin_bit_cast(bf16x8, lf[e + 1]);
-        }
-      }
-      float z[16];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        if (4 * s + u < NB) {
-          normals4(key, path, (uint32_t)(4 * s + u), &z[4 * u]);
-        } else {
-          z[4 * u] = z[4 * u + 1] = z[4 * u + 2] = z[4 * u + 3] = 0.0f;
-        }
-      }
-      uint32_t xh[4], xl[4], yh[4], yl[4];
-      split8(z, xh, xl);
-      split8(z + 8, yh, yl);
-#pragma unroll
-      for (int q4 = 0; q4 < 4; ++q4) {
-        swap32(xh[q4], yh[q4]);
-        swap32(xl[q4], yl[q4]);
-      }
-      const bf16x8 ah = as_bf16x8(xh), al = as_bf16x8(xl);  // paths 0-31
-      const bf16x8 bh = as_bf16x8(yh), bl = as_bf16x8(yl);  // paths 32-63
-#pragma unroll
-      for (int tt = 0; tt < T; ++tt) {
-        const int steps = (tt == T - 1) ? S : (2 * tt + 2 < S ? 2 * tt + 2 : S);
-        if (s < steps) {
-          if (s == 0) {
-            f32x16 init;
-#pragma unroll
-            for (int r4 = 0; r4 < 4; ++r4) {
-              const float4 v = rinit[(32 * tt + 8 * r4 + 4 * g) >> 2];
-              init[4 * r4 + 0] = v.x;
-              init[4 * r4 + 1] = v.y;
-              init[4 * r4 + 2] = v.z;
-              init[4 * r4 + 3] = v.w;
-            }
-            accA[tt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Llo[tt], ah, init, 0, 0, 0);
-            accB[tt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Llo[tt], bh, init, 0, 0, 0);
-          } else {
-            accA[tt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Llo[tt], ah, accA[tt], 0, 0, 0);
-            accB[tt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Llo[tt], bh, accB[tt], 0, 0, 0);
+        for (int tt = 0; tt < T; ++tt) {
+          const int steps = (tt == T - 1) ? S : (2 * tt + 2 < S ? 2 * tt + 2 : S);
+          if (s < steps) {
+            const size_t e = (size_t)(tt * S + s) * 128;
+            Lhi[tt] = __builtin_bit_cast(bf16x8, lf[e]);
+            Llo[tt] = __builtin_bit_cast(bf16x8, lf[e + 1]);
           }
-          accA[tt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Lhi[tt], al, accA[tt], 0, 0, 0);
-          accB[tt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Lhi[tt], bl, accB[tt], 0, 0, 0);
-          accA[tt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Lhi[tt], ah, accA[tt], 0, 0, 0);
-          accB[tt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Lhi[tt], bh, accB[tt], 0, 0, 0);
+        }
+        float z[8];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          const int b = 4 * s + 2 * h + u;
+          if (b < NB) {
+            normals4(key, path, (uint32_t)b, &z[4 * u]);
+          } else {
+            z[4 * u] = z[4 * u + 1] = z[4 * u + 2] = z[4 * u + 3] = 0.0f;
+          }
+        }
+        uint32_t zh[4], zl[4];
+        split8(z, zh, zl);
+        const bf16x8 bh = as_bf16x8(zh), bl = as_bf16x8(zl);
+#pragma unroll
+        for (int tt = 0; tt < T; ++tt) {
+          const int steps = (tt == T - 1) ? S : (2 * tt + 2 < S ? 2 * tt + 2 : S);
+          if (s < steps) {
+            if (s == 0) {
+              f32x16 init;
+#pragma unroll
+              for (int r4 = 0; r4 < 4; ++r4) {
+                const float4 v = rinit[(32 * tt + 8 * r4 + 4 * h) >> 2];
+                init[4 * r4 + 0] = v.x;
+                init[4 * r4 + 1] = v.y;
+                init[4 * r4 + 2] = v.z;
+                init[4 * r4 + 3] = v.w;
+              }
+              acc[tt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Llo[tt], bh, init, 0, 0, 0);
+            } else {
+              acc[tt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Llo[tt], bh, acc[tt], 0, 0, 0);
+            }
+            acc[tt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Lhi[tt], bl, acc[tt], 0, 0, 0);
+            acc[tt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Lhi[tt], bh, acc[tt], 0, 0, 0);
+          }
         }
       }
+      // D: lane l holds column l % 32 (its path), rows 8(r/4) + 4h + r%4
+      uint32_t c = 0;
+#pragma unroll
+      for (int tt = 0; tt < T; ++tt)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const bool excl = tt == T - 1 && r == 15 && h == 1;  // the loss row
+          c += (acc[tt][r] < a.thr && !excl) ? 1u : 0u;
+        }
+      uint32_t c2 = c;
+      swap32(c, c2);  // c = the lower half's counts, c2 = the upper half's, in every lane
+      const int Sp = (int)(c + c2);
+      const float loss = acc[T - 1][15];  // row 32T - 1 in the upper half-wave
+      if (h == 1 && q < a.npaths) tally_path(a, t, q, Sp, loss, false, sh_hist);
     }
-    uint32_t SA = 0, SB = 0;
-#pragma unroll
-    for (int tt = 0; tt < T; ++tt)
-#pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        if (tt == T - 1 && r == 15) {
-          SA += (accA[tt][r] < a.thr && g == 0) ? 1u : 0u;
-          SB += (accB[tt][r] < a.thr && g == 0) ? 1u : 0u;
-        } else {
-          SA += (accA[tt][r] < a.thr) ? 1u : 0u;
-          SB += (accB[tt][r] < a.thr) ? 1u : 0u;
-        }
-      }
-    swap32(SA, SB);
-    const int Sp = (int)(SA + SB);
-    uint32_t la = as_u32(accA[T - 1][15]), lb = as_u32(accB[T - 1][15]);
-    swap32(la, lb);
-    const float loss = as_f32(lb);
-    if (q < a.npaths) tally_path(a, t, q, Sp, loss, false, sh_hist);
   }
 }
 
