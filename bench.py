@@ -124,6 +124,37 @@ def batch_leg(mcr, np):
     return res
 
 
+def accuracy_check(mcr, np):
+    """bf16x3 MFMA vs FP64 on the identical stream (the oracle as checker),
+    on path samples of the n = 50 and n = 600 workloads: relative error of the
+    exceedance probability and of VaR99."""
+    import __graft_entry__ as ge
+
+    orc = ge.load_oracle()
+    threads = min(os.cpu_count() or 1, 16)
+    out = {}
+    for name, G, M in (("n50", dict(n=WORKLOAD["n"], K=WORKLOAD["factors"], T=WORKLOAD["days"],
+                                    ds=WORKLOAD["data_seed"], seed=WORKLOAD["path_seed"],
+                                    k=WORKLOAD["k"], x=WORKLOAD["x"]), 2_000_000),
+                       ("n600", dict(n=LARGE["n"], K=LARGE["factors"], T=LARGE["days"],
+                                     ds=LARGE["data_seed"], seed=LARGE["path_seed"], k=LARGE["k"],
+                                     x=LARGE["x"]), 100_000)):
+        n = G["n"]
+        mu, sigma = mcr.synth_factor_model(n, G["K"], G["T"], G["ds"])
+        r = mcr.run(mu, sigma, G["x"], G["k"], M, G["seed"], precision=mcr.PREC_BF16X3,
+                    alphas=(0.99,))
+        L, *_ = mcr.factor(sigma)
+        ref = orc.simulate_f64(L, mu, np.full(n, 1.0 / n), G["x"], G["seed"], 0, M, G["k"],
+                               want_loss=True, nthreads=threads)
+        rank = min(max(int(np.ceil(0.99 * M)), 1), M)
+        v64 = float(np.sort(ref["loss"])[rank - 1])
+        out[name] = {"paths": M, "kernel": r.kernel, "count_bf16x3": r.count,
+                     "count_fp64": ref["count"],
+                     "rel_err_p_hat": abs(r.count - ref["count"]) / max(ref["count"], 1),
+                     "rel_err_var99": abs(r.var[0] - v64) / abs(v64)}
+    return out
+
+
 def serve_queries(mcr, np, count, seed):
     """Config 5 stream: arrival gaps ~ Exp(1/qps); sizes as config 4; a query
     reuses an earlier query's covariance (and mean) with probability 0.3."""
@@ -309,7 +340,9 @@ def main():
         "value": value, "unit": "paths/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None,
-        "dtype": "bf16x3 (fp32 accumulate)" if is_mfma else "f32",
+        "dtype": "bf16" if is_mfma else "f32",
+        "dtype_detail": ("bf16x3 split products (hi*hi + hi*lo + lo*hi), fp32 accumulation; "
+                         "fp32 normals; FP64 factor") if is_mfma else "fp32; FP64 factor",
         "data": "synthetic (seeded 5-factor model, Philox paths)",
         "config": {"workload": W["name"], "n_assets": n, "paths": W["paths"], "k": W["k"],
                    "x": W["x"], "var_alpha": W["alpha"], "kernel": r.kernel,
@@ -335,6 +368,8 @@ def main():
         L, *_ = mcr.factor(sigma)
         out["cpu_baseline"] = cpu_baseline(L, mu, W["x"], W["k"], W["path_seed"], n,
                                            args.cpu_seconds)
+        out["accuracy_vs_fp64"] = accuracy_check(mcr, np)
+        print(f"# accuracy vs FP64: {out['accuracy_vs_fp64']}", file=sys.stderr, flush=True)
     print(json.dumps(out), flush=True)
 
 
