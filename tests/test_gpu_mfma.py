@@ -40,9 +40,13 @@ def test_mfma_vs_fp64_same_stream(mcr, orc, n):
 
 
 @pytest.mark.parametrize("n,M", [(5, 100_000), (50, 100_000), (128, 40_000), (200, 20_000),
-                                 (333, 10_000), (600, 6_000)])
+                                 (333, 10_000), (450, 8_000), (520, 6_000), (600, 6_000),
+                                 (608, 5_000)])
 def test_mfma_lds_vs_fp64_same_stream(mcr, orc, n, M):
-    """Large-n MFMA kernel (normals staged in LDS), same tolerances."""
+    """Large-n MFMA kernel (normals staged in LDS), same tolerances.  The sizes
+    cover every workgroup shape (T = 1, 2, 5, 7, 11, 15 row tiles with four
+    waves; 17, 19 with eight), an odd step count with eight waves (n = 520:
+    S = 33, the last phase half full) and the largest n."""
     seed, x = 91, 0.02
     k = max(1, n // 5)
     mu, sigma = factor_like(n, min(20, n), 70 + n, T=max(4000, 2 * n))
