@@ -265,12 +265,22 @@ def large_leg(mcr, np):
         res[name] = {"kernel": r.kernel, "kernel_ms": ms, "paths_per_s": G["paths"] / (ms * 1e-3),
                      "tflops": G["n"] * (G["n"] + 1) * G["paths"] / (ms * 1e-3) / 1e12,
                      "count": r.count, "p_hat": r.p_hat, "var99": r.var[0], "var999": r.var[1]}
+    # k = 10 is not a tail event at n = 600 (P ~ 0.6): also the smallest k with
+    # P(S >= k) <= 1e-2, read off the in-kernel S histogram of the same stream
+    h = mcr.run(mu, sigma, G["x"], G["k"], G["paths"], G["path_seed"], precision=mcr.PREC_BF16X3,
+                want_hist=True)
+    tail = np.cumsum(h.s_hist[::-1])[::-1] / G["paths"]
+    kt = int(np.argmax(tail <= 1e-2))
+    t = mcr.run(mu, sigma, G["x"], kt, G["paths"], G["path_seed"], precision=mcr.PREC_BF16X3)
+    res["tail_k"] = {"k": kt, "count": t.count, "p_hat": t.p_hat, "std_error": t.std_error,
+                     "ci": [t.ci_lower, t.ci_upper], "kernel_ms": t.ms_kernel,
+                     "count_equals_histogram_tail": t.count == int(h.s_hist[kt:].sum())}
     f, b = res["fp32"], res["bf16x3"]
     res["speedup_bf16x3_vs_fp32"] = f["kernel_ms"] / b["kernel_ms"]
     res["rel_diff_p_hat"] = abs(b["p_hat"] - f["p_hat"]) / f["p_hat"]
     res["rel_diff_var99"] = abs(b["var99"] - f["var99"]) / f["var99"]
-    print(f"# n=600: fp32 {f['kernel_ms']:.2f} ms, bf16x3 {b['kernel_ms']:.2f} ms", file=sys.stderr,
-          flush=True)
+    print(f"# n=600: fp32 {f['kernel_ms']:.2f} ms, bf16x3 {b['kernel_ms']:.2f} ms; tail k={kt}: "
+          f"P={t.p_hat:.5f} +- {t.std_error:.5f}", file=sys.stderr, flush=True)
     return res
 
 

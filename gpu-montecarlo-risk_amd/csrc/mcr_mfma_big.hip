@@ -70,7 +70,10 @@ __device__ __forceinline__ void mfma_lds_groups(const PathArgs& a, uint64_t g0, 
                                                 uint64_t gstep, BigShared<NW>& sh, LaneTally& t,
                                                 unsigned* sh_hist) {
   constexpr int P = NW / 4;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 5;
+  // the wave index is uniform; readfirstlane lets the compiler see it, so
+  // the tile deal and every per-tile step test are scalar (no exec masks)
+  const int lane = threadIdx.x & 63, g = lane >> 5;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int n = a.n, T = a.T, S = a.S;
   const int nb = (n + 3) >> 2;
   const int nph = (S + P - 1) / P;
@@ -88,7 +91,7 @@ __device__ __forceinline__ void mfma_lds_groups(const PathArgs& a, uint64_t g0, 
   const int loss_wave = (loss_slot & 1) ? NW - 1 - (T - 1) % NW : (T - 1) % NW;
 
   // generator role of this thread: path rp, block ru of step sp of a phase
-  const int rp = threadIdx.x & 63, ru = (threadIdx.x >> 6) & 3, sp = threadIdx.x >> 8;
+  const int rp = threadIdx.x & 63, ru = wave & 3, sp = wave >> 2;
   for (uint64_t grp = g0; grp < g1; grp += gstep) {
     const uint64_t rpath = a.path0 + grp * 64 + rp;
     if (sp < S) gen_step<NW>(&sh.zt, 0, sp, sp, nb, key, rpath, rp, ru);
@@ -158,9 +161,11 @@ __device__ __forceinline__ void mfma_lds_groups(const PathArgs& a, uint64_t g0, 
             accB[i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Lhi, bh, accB[i], 0, 0, 0);
           }
         }
+        // unconditional, so that the copies are renamings of the unrolled
+        // steps (a conditional copy is a move that waits for the load)
+        if (PF) {
 #pragma unroll
-        for (int i = 0; i < TW; ++i) {
-          if (PF && s + 1 < steps[i]) {
+          for (int i = 0; i < TW; ++i) {
             Lh[i] = Nh[i];
             Ll[i] = Nl[i];
           }

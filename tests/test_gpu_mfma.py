@@ -62,6 +62,20 @@ def test_mfma_lds_vs_fp64_same_stream(mcr, orc, n, M):
     assert abs(r.var[0] - v64) <= 1e-4 * abs(v64), (r.var[0], v64)
 
 
+@pytest.mark.parametrize("n,M,kernel", [(50, 300_000, 3), (50, 200_000, 4), (600, 50_000, 4)])
+def test_mfma_s_histogram_matches_per_path_counts(mcr, n, M, kernel):
+    """The in-kernel (n+1)-bin histogram of S equals the histogram of the
+    per-path S the same launch stores, and its tail above k is the count."""
+    mu, sigma = factor_like(n, min(20, n), 500 + n, T=max(4000, 2 * n))
+    k = 10
+    r = mcr.run(mu, sigma, 0.02, k, M, 17, precision=mcr.PREC_BF16X3, kernel=kernel,
+                want_S=True, want_hist=True)
+    assert r.s_hist.shape == (n + 1,)
+    assert np.array_equal(r.s_hist, np.bincount(r.S, minlength=n + 1))
+    assert int(r.s_hist[k:].sum()) == r.count
+    assert int(r.s_hist.sum()) == M
+
+
 def test_mfma_kernels_agree(mcr):
     """Small-n MFMA (register operands) and LDS MFMA kernels: same products,
     summation order may differ in the last fp32 bit, so counts agree to a few
