@@ -96,16 +96,73 @@ __device__ __forceinline__ void mfma_lds_groups(const PathArgs& a, uint64_t g0, 
     const uint64_t rpath = a.path0 + grp * 64 + rp;
     if (sp < S) gen_step<NW>(&sh.zt, 0, sp, sp, nb, key, rpath, rp, ru);
     __syncthreads();
+    // accumulators start at the rows' initial values (+inf pads, -w^T mu):
+    // loaded here without the fragment prefetch, at the tile's first step
+    // with it (measured: each placement is the faster one for its shapes)
     f32x16 accA[TW], accB[TW];
-    // factor fragments of the current K step; the next step's are loaded
-    // before this step's MFMAs (also across the phase barrier)
-    uint4 Lh[TW], Ll[TW];
+    auto init_acc = [&](int i) {
+#pragma unroll
+      for (int r4 = 0; r4 < 4; ++r4) {
+        const float4 v = rinit[(32 * tile[i] + 8 * r4 + 4 * g) >> 2];
+        accA[i][4 * r4 + 0] = v.x;
+        accA[i][4 * r4 + 1] = v.y;
+        accA[i][4 * r4 + 2] = v.z;
+        accA[i][4 * r4 + 3] = v.w;
+      }
+      accB[i] = accA[i];
+    };
+    if (!PF) {
+#pragma unroll
+      for (int i = 0; i < TW; ++i)
+        if (steps[i] > 0) init_acc(i);
+    }
+    // One K step s (slot q of the phase in LDS buffer cb) with this step's
+    // factor fragments ch/cl; with PF the next step's are loaded into nh/nl
+    // before the MFMAs are issued.
+    auto k_step = [&](int s, int q, int cb, const uint4 (&ch)[TW], const uint4 (&cl)[TW],
+                      uint4 (&nh)[TW], uint4 (&nl)[TW]) {
+#pragma unroll
+      for (int i = 0; i < TW; ++i) {
+        if (PF && s + 1 < steps[i]) {
+          const size_t e = (size_t)(tile[i] * S + s + 1) * 128;
+          nh[i] = lf[e];
+          nl[i] = lf[e + 1];
+        }
+      }
+      const bf16x8 ah = __builtin_bit_cast(bf16x8, sh.zt.v[cb][q][0][0][lane]);
+      const bf16x8 al = __builtin_bit_cast(bf16x8, sh.zt.v[cb][q][0][1][lane]);
+      const bf16x8 bh = __builtin_bit_cast(bf16x8, sh.zt.v[cb][q][1][0][lane]);
+      const bf16x8 bl = __builtin_bit_cast(bf16x8, sh.zt.v[cb][q][1][1][lane]);
+#pragma unroll
+      for (int i = 0; i < TW; ++i) {
+        if (s < steps[i]) {
+          uint4 h = ch[i], l = cl[i];
+          if (!PF) {
+            const size_t e = (size_t)(tile[i] * S + s) * 128;
+            h = lf[e];
+            l = lf[e + 1];
+          }
+          const bf16x8 Lhi = __builtin_bit_cast(bf16x8, h);
+          const bf16x8 Llo = __builtin_bit_cast(bf16x8, l);
+          if (PF && s == 0) init_acc(i);
+          accA[i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Llo, ah, accA[i], 0, 0, 0);
+          accB[i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Llo, bh, accB[i], 0, 0, 0);
+          accA[i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Lhi, al, accA[i], 0, 0, 0);
+          accB[i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Lhi, bl, accB[i], 0, 0, 0);
+          accA[i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Lhi, ah, accA[i], 0, 0, 0);
+          accB[i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Lhi, bh, accB[i], 0, 0, 0);
+        }
+      }
+    };
+    // factor fragments of the current K step (A) and, with PF, of the next
+    // (B, loaded before this step's MFMAs, also across the phase barrier)
+    uint4 LAh[TW], LAl[TW], LBh[TW], LBl[TW];
 #pragma unroll
     for (int i = 0; i < TW; ++i) {
       if (PF && steps[i] > 0) {
         const size_t e = (size_t)(tile[i] * S) * 128;
-        Lh[i] = lf[e];
-        Ll[i] = lf[e + 1];
+        LAh[i] = lf[e];
+        LAl[i] = lf[e + 1];
       }
     }
     for (int ph = 0; ph < nph; ++ph) {
@@ -116,58 +173,14 @@ __device__ __forceinline__ void mfma_lds_groups(const PathArgs& a, uint64_t g0, 
       for (int q = 0; q < P; ++q) {
         const int s = ph * P + q;
         if (s >= S) break;
-        uint4 Nh[TW], Nl[TW];
-#pragma unroll
-        for (int i = 0; i < TW; ++i) {
-          if (PF && s + 1 < steps[i]) {
-            const size_t e = (size_t)(tile[i] * S + s + 1) * 128;
-            Nh[i] = lf[e];
-            Nl[i] = lf[e + 1];
-          }
-        }
-        const bf16x8 ah = __builtin_bit_cast(bf16x8, sh.zt.v[cb][q][0][0][lane]);
-        const bf16x8 al = __builtin_bit_cast(bf16x8, sh.zt.v[cb][q][0][1][lane]);
-        const bf16x8 bh = __builtin_bit_cast(bf16x8, sh.zt.v[cb][q][1][0][lane]);
-        const bf16x8 bl = __builtin_bit_cast(bf16x8, sh.zt.v[cb][q][1][1][lane]);
-#pragma unroll
-        for (int i = 0; i < TW; ++i) {
-          if (s < steps[i]) {
-            if (!PF) {
-              const size_t e = (size_t)(tile[i] * S + s) * 128;
-              Lh[i] = lf[e];
-              Ll[i] = lf[e + 1];
-            }
-            const bf16x8 Lhi = __builtin_bit_cast(bf16x8, Lh[i]);
-            const bf16x8 Llo = __builtin_bit_cast(bf16x8, Ll[i]);
-            if (s == 0) {
-              f32x16 init;
-#pragma unroll
-              for (int r4 = 0; r4 < 4; ++r4) {
-                const float4 v = rinit[(32 * tile[i] + 8 * r4 + 4 * g) >> 2];
-                init[4 * r4 + 0] = v.x;
-                init[4 * r4 + 1] = v.y;
-                init[4 * r4 + 2] = v.z;
-                init[4 * r4 + 3] = v.w;
-              }
-              accA[i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Llo, ah, init, 0, 0, 0);
-              accB[i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Llo, bh, init, 0, 0, 0);
-            } else {
-              accA[i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Llo, ah, accA[i], 0, 0, 0);
-              accB[i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Llo, bh, accB[i], 0, 0, 0);
-            }
-            accA[i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Lhi, al, accA[i], 0, 0, 0);
-            accB[i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Lhi, bl, accB[i], 0, 0, 0);
-            accA[i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Lhi, ah, accA[i], 0, 0, 0);
-            accB[i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Lhi, bh, accB[i], 0, 0, 0);
-          }
-        }
+        k_step(s, q, cb, LAh, LAl, LBh, LBl);
         // unconditional, so that the copies are renamings of the unrolled
         // steps (a conditional copy is a move that waits for the load)
         if (PF) {
 #pragma unroll
           for (int i = 0; i < TW; ++i) {
-            Lh[i] = Nh[i];
-            Ll[i] = Nl[i];
+            LAh[i] = LBh[i];
+            LAl[i] = LBl[i];
           }
         }
       }
@@ -213,8 +226,16 @@ __device__ __forceinline__ void mfma_lds_groups(const PathArgs& a, uint64_t g0, 
   }
 }
 
+// Waves per SIMD the register budget must allow (measured best): three for
+// the 3-tile 4-wave shape (it lands within a few registers of 168, and the
+// allocation granularity of 8 decides between two and three), four for the
+// 2-tile shape (128 VGPRs, a few spilled).
+constexpr int big_min_waves(int TW, int NW) {
+  return NW == 4 && TW == 3 ? 3 : (NW == 4 && TW == 2 ? 4 : 1);
+}
+
 template <int TW, int NW, bool PF>
-__global__ __launch_bounds__(64 * NW) void k_mfma_big(PathArgs a) {
+__global__ __launch_bounds__(64 * NW, big_min_waves(TW, NW)) void k_mfma_big(PathArgs a) {
   __shared__ BigShared<NW> sh;
   __shared__ unsigned sh_hist[kMaxN + 1];
   const bool want_hist = a.s_hist != nullptr;
@@ -239,7 +260,7 @@ __global__ __launch_bounds__(64 * NW) void k_mfma_big(PathArgs a) {
 // workgroup leaves the loop.  Tallies are flushed into the item's query after
 // each item.
 template <int TW, int NW, bool PF>
-__global__ __launch_bounds__(64 * NW) void k_batch_paths(BatchArgs b) {
+__global__ __launch_bounds__(64 * NW, big_min_waves(TW, NW)) void k_batch_paths(BatchArgs b) {
   __shared__ BigShared<NW> sh;
   __shared__ unsigned s_item;
   for (;;) {
