@@ -6,7 +6,7 @@
 // with the same augmented rows as the small MFMA kernel (assets, +inf pads,
 // loss row 32T-1 = -v, initial -w^T mu) and the same bf16 hi/lo split.
 //
-// A workgroup of NW waves (4, or 8 for n >= 512) owns 64 paths.  The 64 x 32T
+// A workgroup of NW waves (4, or 8 for n >= 384) owns 64 paths.  The 64 x 32T
 // accumulator block is spread over the waves' VGPRs: row tile t belongs to
 // wave (t/NW even ? t%NW : NW-1-t%NW) (snake order, since tile t needs
 // min(2t+2, S) steps), at most TW tiles per wave; the shape keeps at least two
@@ -229,9 +229,9 @@ __device__ __forceinline__ void mfma_lds_groups(const PathArgs& a, uint64_t g0, 
 // Waves per SIMD the register budget must allow (measured best): three for
 // the 3-tile 4-wave shape (it lands within a few registers of 168, and the
 // allocation granularity of 8 decides between two and three), four for the
-// 2-tile shape (128 VGPRs, a few spilled).
+// 2-tile shapes (128 VGPRs, a few spilled; two 8-wave workgroups per CU).
 constexpr int big_min_waves(int TW, int NW) {
-  return NW == 4 && TW == 3 ? 3 : (NW == 4 && TW == 2 ? 4 : 1);
+  return NW == 4 && TW == 3 ? 3 : (TW == 2 ? 4 : 1);
 }
 
 template <int TW, int NW, bool PF>
@@ -295,19 +295,20 @@ hipError_t launch_batch_kernel(BatchKernel k, const BatchArgs& b, int grid, int 
   return hipGetLastError();
 }
 
-// Shape of the LDS MFMA kernel for T row tiles: four waves (several
-// workgroups per CU, so one's barrier waits are covered by another's work)
-// while 4 tiles per wave fit in half the register file; eight waves with 3
-// tiles each beyond that, where four waves would need 5 tiles and run one
-// wave per SIMD.  Fragment prefetch where it keeps the occupancy (measured:
-// -16 % at n = 600; +14..60 % where it cost a wave per SIMD).
-int big_waves(int T) { return T <= 16 ? 4 : 8; }
+// Shape of the LDS MFMA kernel for T row tiles (measured best per range):
+// four waves with up to 3 tiles each (several workgroups per CU, so one's
+// barrier waits are covered by another's work); eight waves with 2 tiles
+// each at four waves per SIMD for 13-16 tiles (one wave per SIMD more than
+// four waves with 4 tiles); eight waves with 3 tiles each beyond, with the
+// next-step fragment prefetch (-16 % at n = 600; it costs the other shapes a
+// wave per SIMD).
+int big_waves(int T) { return T <= 12 ? 4 : 8; }
 
 BatchKernel batch_big_kernel(int T) {
   if (T <= 4) return k_batch_paths<1, 4, true>;
   if (T <= 8) return k_batch_paths<2, 4, false>;
   if (T <= 12) return k_batch_paths<3, 4, false>;
-  if (T <= 16) return k_batch_paths<4, 4, false>;
+  if (T <= 16) return k_batch_paths<2, 8, false>;
   return T <= 24 ? k_batch_paths<3, 8, true> : nullptr;
 }
 
@@ -315,7 +316,7 @@ PathKernel mfma_big_kernel(int T) {
   if (T <= 4) return k_mfma_big<1, 4, true>;
   if (T <= 8) return k_mfma_big<2, 4, false>;
   if (T <= 12) return k_mfma_big<3, 4, false>;
-  if (T <= 16) return k_mfma_big<4, 4, false>;
+  if (T <= 16) return k_mfma_big<2, 8, false>;
   return T <= 24 ? k_mfma_big<3, 8, true> : nullptr;
 }
 

@@ -44,9 +44,10 @@ def test_mfma_vs_fp64_same_stream(mcr, orc, n):
                                  (608, 5_000)])
 def test_mfma_lds_vs_fp64_same_stream(mcr, orc, n, M):
     """Large-n MFMA kernel (normals staged in LDS), same tolerances.  The sizes
-    cover every workgroup shape (T = 1, 2, 5, 7, 11, 15 row tiles with four
-    waves; 17, 19 with eight), an odd step count with eight waves (n = 520:
-    S = 33, the last phase half full) and the largest n."""
+    cover every workgroup shape (T = 1, 2, 5, 7, 11 row tiles with four
+    waves; 15 with eight waves of 2 tiles; 17, 19, 20 with eight of 3 and the
+    fragment prefetch), an odd step count with eight waves (n = 520: S = 33,
+    the last phase half full) and the largest n."""
     seed, x = 91, 0.02
     k = max(1, n // 5)
     mu, sigma = factor_like(n, min(20, n), 70 + n, T=max(4000, 2 * n))
