@@ -76,7 +76,8 @@ void set_windows(const mcr_query* q, int n_alpha, FactorArgs* f);
 
 // Persistent batch classes: queries with n <= kSmallMaxN run the per-wave
 // register-operand MFMA kernel of their (NB, T), larger ones the LDS MFMA
-// kernel of their TW.  Items of one class are contiguous and launched together.
+// kernel of their shape.  Items of one class are contiguous and launched
+// together.
 constexpr int kSmallMaxN = 127;
 struct ClassRun {
   BatchKernel k;

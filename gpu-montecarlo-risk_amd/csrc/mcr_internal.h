@@ -131,7 +131,8 @@ struct BatchArgs {
 
 // Persistent batch kernels, one per class: n <= 127 runs the register-operand
 // MFMA code per wave (class = Philox blocks NB, tiles T); larger n runs the
-// LDS MFMA code per workgroup (class = TW = ceil(T / 4) tile slots per wave).
+// LDS MFMA code per workgroup (class = the workgroup shape chosen for T: waves
+// per workgroup and row-tile slots per wave).
 typedef void (*BatchKernel)(BatchArgs);
 BatchKernel batch_small_kernel(int n, int T);  // n <= 127, 256 threads
 BatchKernel batch_big_kernel(int T);           // T <= 24, 64 * big_waves(T) threads
