@@ -421,17 +421,21 @@ hipError_t launch_prep(const PrepArgs& a, hipStream_t stream) {
 }
 
 // ---------------------------------------------------------------------------
-// Tile kernel: 64 paths per workgroup (lane = path), 4 waves.  Phase 1 writes
+// Tile kernel: 64 paths per workgroup (lane = path), NW waves.  Phase 1 writes
 // the paths' normals to LDS as float4 [block][lane]; phase 2 splits the rows
 // into 8-row register blocks dealt to the waves in snake order (block cost grows
 // with the row index), the loss row being one more block.  Row i still sees
 // mu_i, then z_0 .. z_i in order, so results equal the register kernel's.
+// The normals of 64 paths take up to 152 KiB, so at large n one workgroup
+// fills a CU's LDS: NW grows as fewer workgroups fit (4, 8, 16 waves), keeping
+// several waves per SIMD to cover the scalar factor loads and LDS reads.
 // ---------------------------------------------------------------------------
-constexpr int kTileStaticLds = 4 * 64 * 4 + 64 * 4 + (kMaxN + 1) * 4;
+constexpr int kTileStaticLds = 16 * 64 * 4 + 64 * 4 + (kMaxN + 1) * 4;
 
-__global__ __launch_bounds__(256) void k_valu_tile(PathArgs a) {
+template <int NW>
+__global__ __launch_bounds__(64 * NW) void k_valu_tile(PathArgs a) {
   extern __shared__ float4 zs[];  // [nb][64]
-  __shared__ int sS[4][64];
+  __shared__ int sS[NW][64];
   __shared__ float sLoss[64];
   __shared__ unsigned sh_hist[kMaxN + 1];
   const int n = a.n;
@@ -455,7 +459,7 @@ __global__ __launch_bounds__(256) void k_valu_tile(PathArgs a) {
   for (uint64_t tile = blockIdx.x; tile * kTilePaths < a.npaths; tile += gridDim.x) {
     const uint64_t q = tile * kTilePaths + lane;
     const uint64_t path = a.path0 + q;
-    for (int b = wave; b < nb; b += 4) {
+    for (int b = wave; b < nb; b += NW) {
       float z[4];
       normals4(key, path, (uint32_t)b, z);
       zs[b * 64 + lane] = make_float4(z[0], z[1], z[2], z[3]);
@@ -465,8 +469,8 @@ __global__ __launch_bounds__(256) void k_valu_tile(PathArgs a) {
     int S = 0;
     bool near = false;
     for (int rb = 0; rb <= nrb; ++rb) {
-      const int p = rb >> 2, r4 = rb & 3;
-      const int owner = (p & 1) ? 3 - r4 : r4;
+      const int p = rb / NW, rw = rb % NW;
+      const int owner = (p & 1) ? NW - 1 - rw : rw;
       if (owner != wave) continue;
       if (rb == nrb) {
         // loss row: g = w^T mu + sum_j v_j z_j, j ascending
@@ -537,7 +541,7 @@ __global__ __launch_bounds__(256) void k_valu_tile(PathArgs a) {
       int tot = 0;
       bool nr = false;
 #pragma unroll
-      for (int w = 0; w < 4; ++w) {
+      for (int w = 0; w < NW; ++w) {
         const int e = sS[w][lane];
         tot += e & 0xffff;
         nr |= (e >> 30) & 1;
@@ -616,16 +620,25 @@ hipError_t launch_paths(int kernel, const PathArgs& a, int num_cus, hipStream_t 
     if (!attr) {
       // the largest dynamic request (n = kMaxN); static LDS comes on top
       const int max_dyn = ((kMaxN + 3) / 4) * 64 * (int)sizeof(float4);
-      hipError_t e = hipFuncSetAttribute((const void*)k_valu_tile,
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, max_dyn);
-      if (e != hipSuccess) return e;
+      for (const void* f : {(const void*)k_valu_tile<4>, (const void*)k_valu_tile<8>,
+                            (const void*)k_valu_tile<16>}) {
+        hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, max_dyn);
+        if (e != hipSuccess) return e;
+      }
       attr = true;
     }
     uint64_t per_cu = (160 * 1024) / (lds + kTileStaticLds);
     if (per_cu < 1) per_cu = 1;
     if (per_cu > 8) per_cu = 8;
     const uint64_t blocks = min_u64((a.npaths + kTilePaths - 1) / kTilePaths, (uint64_t)num_cus * per_cu);
-    hipLaunchKernelGGL(k_valu_tile, dim3((unsigned)blocks), dim3(256), lds, stream, a);
+    // waves per workgroup: about 16 per CU where the LDS limits the
+    // workgroups, at most 8 per SIMD
+    if (per_cu == 1)
+      hipLaunchKernelGGL(k_valu_tile<16>, dim3((unsigned)blocks), dim3(1024), lds, stream, a);
+    else if (per_cu <= 3)
+      hipLaunchKernelGGL(k_valu_tile<8>, dim3((unsigned)blocks), dim3(512), lds, stream, a);
+    else
+      hipLaunchKernelGGL(k_valu_tile<4>, dim3((unsigned)blocks), dim3(256), lds, stream, a);
     return hipGetLastError();
   }
   if (kernel == 4) {
