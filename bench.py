@@ -357,10 +357,13 @@ def main():
         "config": {"workload": W["name"], "n_assets": n, "paths": W["paths"], "k": W["k"],
                    "x": W["x"], "var_alpha": W["alpha"], "kernel": r.kernel,
                    "parallelism": "single GPU"},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": TRAFFIC_BYTES.get(r.kernel),
-                     "kernel": r.kernel, "kernel_ms": k_ms,
-                     "flop_per_path": n * (n + 1)},
+        "roofline": {"bound": "mfma" if is_mfma else "valu", "achieved": achieved, "peak": peak,
+                     "unit": "TFLOP/s", "frac": achieved / peak,
+                     "traffic": TRAFFIC_BYTES.get(r.kernel), "kernel": r.kernel, "kernel_ms": k_ms,
+                     "flop_per_path": n * (n + 1),
+                     "note": ("achieved counts the mat-vec FLOPs only; the kernel is bound by the "
+                              "VALU issue of normal generation (Philox4x32-10, Box-Muller: "
+                              "~83 % of its instructions), see profiles/k_mfma_n50_100M.md")},
         "result": {"count": r.count, "p_hat": r.p_hat, "var99": r.var[0], "es99": r.es[0]},
         "wall_ms_per_step": wall_ms,
         "wall_paths_per_s": W["paths"] / (wall_ms * 1e-3),
