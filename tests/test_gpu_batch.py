@@ -17,7 +17,7 @@ from helpers import factor_like
 pytestmark = pytest.mark.gpu
 
 
-def portfolios(seed=0, sizes=(5, 17, 50, 64, 100, 128, 200, 333, 600), paths=None):
+def portfolios(seed=0, sizes=(1, 5, 17, 50, 64, 100, 128, 200, 333, 450, 600, 608), paths=None):
     rng = np.random.default_rng(seed)
     out = []
     for i, n in enumerate(sizes):
@@ -43,7 +43,7 @@ def same(a, b):
 
 
 def test_batch_equals_alone(mcr):
-    ps = portfolios(1, paths=[1, 63, 64, 65, 20000, 12345, 30000, 5000, 4000])
+    ps = portfolios(1, paths=[1, 63, 64, 65, 20000, 12345, 30000, 5000, 4000, 7000, 3000, 2500])
     res, o = mcr.run_batch(ps, use_graph=False, chunk_paths=1024)
     assert o.work_items >= len(ps)
     for p, r in zip(ps, res):

@@ -19,7 +19,7 @@ from helpers import factor_like
 pytestmark = pytest.mark.gpu
 
 
-def stream(seed, count, sizes=(5, 12, 33, 50, 64, 100, 200, 333, 600), reuse=0.3,
+def stream(seed, count, sizes=(1, 5, 12, 33, 50, 64, 100, 200, 333, 450, 600, 608), reuse=0.3,
            paths=(2_000, 80_000)):
     """Queries of mixed sizes; a fraction reuses an earlier covariance."""
     rng = np.random.default_rng(seed)
