@@ -287,8 +287,9 @@ def large_leg(mcr, np):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the path shards by "
-              "path-id range, see DESIGN.md", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); across GPUs the paths "
+              "would shard by path-id range (mcr_query.path_offset) with one all-reduce of "
+              "the integer tallies", file=sys.stderr)
         sys.exit(2)
     import numpy as np
 
