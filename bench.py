@@ -314,8 +314,15 @@ def main():
         return mcr.run(mu, sigma, W["x"], W["k"], W["paths"], W["path_seed"], precision=precision,
                        alphas=(W["alpha"],))
 
-    for _ in range(args.warmup):
+    # the first warm-up query pays the library's lazy set-up (HIP context,
+    # code objects, device arena, pinned staging): the cold figure a driver
+    # that calls run_model once per process sees
+    cold_ms = None
+    for i in range(args.warmup):
+        t_w = time.perf_counter()
         step(prec)
+        if i == 0:
+            cold_ms = (time.perf_counter() - t_w) * 1e3
     kms, dms = [], []
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -367,6 +374,7 @@ def main():
                               "~83 % of its instructions), see profiles/k_mfma_n50_100M.md")},
         "result": {"count": r.count, "p_hat": r.p_hat, "var99": r.var[0], "es99": r.es[0]},
         "wall_ms_per_step": wall_ms,
+        "cold_first_query_ms": cold_ms,
         "wall_paths_per_s": W["paths"] / (wall_ms * 1e-3),
         "kernel_paths_per_s": W["paths"] / (k_ms * 1e-3),
         "normals_per_s": W["paths"] * n / (k_ms * 1e-3),
