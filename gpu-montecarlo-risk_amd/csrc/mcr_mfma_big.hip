@@ -222,7 +222,10 @@ __device__ __forceinline__ void mfma_lds_groups(const PathArgs& a, uint64_t g0, 
       for (int w = 0; w < NW; ++w) Sp += (int)sh.sS[w][lane];
       tally_path(a, t, q, Sp, sh.sLoss[lane], false, sh_hist);
     }
-    __syncthreads();
+    // no barrier here: sS and sLoss are next written at the end of the next
+    // group, after its phase barriers, which wave 0 reaches only once its
+    // tally is done; the next group's first writes go to the operand tiles,
+    // whose last reads preceded the final phase barrier
   }
 }
 
