@@ -6,7 +6,7 @@
 // with the same augmented rows as the small MFMA kernel (assets, +inf pads,
 // loss row 32T-1 = -v, initial -w^T mu) and the same bf16 hi/lo split.
 //
-// A workgroup of NW waves (4, or 8 for n >= 384) owns 64 paths.  The 64 x 32T
+// A workgroup of NW waves (4, or 8 for 384 <= n < 512) owns 64 paths.  The 64 x 32T
 // accumulator block is spread over the waves' VGPRs: row tile t belongs to
 // wave (t/NW even ? t%NW : NW-1-t%NW) (snake order, since tile t needs
 // min(2t+2, S) steps), at most TW tiles per wave; the shape keeps at least two
@@ -15,7 +15,8 @@
 // Philox block (4 normals of one path) and writes its bf16 hi/lo parts
 // straight into a double-buffered LDS tile already in B-operand order, so the
 // next phase is generated while this one is multiplied; factor fragments
-// stream from L2.
+// stream from L2.  For n >= 512 (17-24 row tiles) the roles are split
+// instead: 12 waves only multiply, 4 only generate (k_mfma_spec below).
 #include "mcr_kern_common.h"
 
 namespace mcr {
@@ -283,6 +284,193 @@ __global__ __launch_bounds__(64 * NW, big_min_waves(TW, NW)) void k_batch_paths(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Specialised form: NC consumer waves hold the accumulators (64 paths, up to
+// TW row tiles each) and only multiply; 4 producer waves only generate the
+// next phase's normals (a phase is P K steps: P Philox blocks per producer
+// lane).  A SIMD runs one producer beside NC/4 consumers, so generation and
+// MFMAs overlap within every phase instead of alternating in every wave.
+// ---------------------------------------------------------------------------
+constexpr int kProducers = 4;
+
+template <int NC, int P>
+struct SpecShared {
+  ZTile<4 * P> zt;  // P steps per phase, double-buffered
+  uint32_t sS[NC][64];
+  float sLoss[64];
+};
+
+template <int NC, int TW, int P>
+__device__ __forceinline__ void mfma_spec_groups(const PathArgs& a, uint64_t g0, uint64_t g1,
+                                                 uint64_t gstep, SpecShared<NC, P>& sh,
+                                                 LaneTally& t, unsigned* sh_hist) {
+  const int lane = threadIdx.x & 63, g = lane >> 5;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int n = a.n, T = a.T, S = a.S;
+  const int nb = (n + 3) >> 2;
+  const int nph = (S + P - 1) / P;
+  const Key2 key = key_of(a.seed);
+
+  // The two roles run separate loops with the same barriers per group
+  // (1 + nph + 1), so neither keeps the other's registers live.
+  if (wave >= NC) {
+    // producer lane: path rp, blocks kP .. kP+P-1 of the phase's 4P per path
+    // (block j: step j / 4 of the phase, Philox block j % 4 of that step)
+    const int rp = lane, k0 = P * (wave - NC);
+    for (uint64_t grp = g0; grp < g1; grp += gstep) {
+      const uint64_t rpath = a.path0 + grp * 64 + rp;
+#pragma unroll
+      for (int u = 0; u < P; ++u) {
+        const int j = k0 + u, sp = j >> 2;
+        if (sp < S) gen_step<4 * P>(&sh.zt, 0, sp, sp, nb, key, rpath, rp, j & 3);
+      }
+      __syncthreads();
+      for (int ph = 0; ph < nph; ++ph) {
+        if (ph + 1 < nph) {
+#pragma unroll
+          for (int u = 0; u < P; ++u) {
+            const int j = k0 + u, sp = j >> 2, sn = (ph + 1) * P + sp;
+            if (sn < S) gen_step<4 * P>(&sh.zt, (ph + 1) & 1, sp, sn, nb, key, rpath, rp, j & 3);
+          }
+        }
+        __syncthreads();
+      }
+      __syncthreads();  // the consumers' count barrier
+    }
+    return;
+  }
+
+  const uint4* __restrict__ lf = a.lfrag + lane * 2;
+  const float4* __restrict__ rinit = (const float4*)a.rowinit;
+  int tile[TW], steps[TW];
+#pragma unroll
+  for (int i = 0; i < TW; ++i) {
+    tile[i] = NC * i + ((i & 1) ? NC - 1 - wave : wave);
+    steps[i] = tile[i] >= T ? 0 : (tile[i] == T - 1 ? S : (2 * tile[i] + 2 < S ? 2 * tile[i] + 2 : S));
+  }
+  const int loss_slot = (T - 1) / NC;
+  const int loss_wave = (loss_slot & 1) ? NC - 1 - (T - 1) % NC : (T - 1) % NC;
+  for (uint64_t grp = g0; grp < g1; grp += gstep) {
+    __syncthreads();  // the first phase's normals are in
+    f32x16 accA[TW], accB[TW];
+#pragma unroll
+    for (int i = 0; i < TW; ++i) {
+      if (steps[i] > 0) {
+#pragma unroll
+        for (int r4 = 0; r4 < 4; ++r4) {
+          const float4 v = rinit[(32 * tile[i] + 8 * r4 + 4 * g) >> 2];
+          accA[i][4 * r4 + 0] = v.x;
+          accA[i][4 * r4 + 1] = v.y;
+          accA[i][4 * r4 + 2] = v.z;
+          accA[i][4 * r4 + 3] = v.w;
+        }
+        accB[i] = accA[i];
+      }
+    }
+    for (int ph = 0; ph < nph; ++ph) {
+      const int cb = ph & 1;
+#pragma unroll
+      for (int q = 0; q < P; ++q) {
+        const int s = ph * P + q;
+        if (s >= S) break;
+        const bf16x8 ah = __builtin_bit_cast(bf16x8, sh.zt.v[cb][q][0][0][lane]);
+        const bf16x8 al = __builtin_bit_cast(bf16x8, sh.zt.v[cb][q][0][1][lane]);
+        const bf16x8 bh = __builtin_bit_cast(bf16x8, sh.zt.v[cb][q][1][0][lane]);
+        const bf16x8 bl = __builtin_bit_cast(bf16x8, sh.zt.v[cb][q][1][1][lane]);
+#pragma unroll
+        for (int i = 0; i < TW; ++i) {
+          if (s < steps[i]) {
+            const size_t e = (size_t)(tile[i] * S + s) * 128;
+            const bf16x8 Lhi = __builtin_bit_cast(bf16x8, lf[e]);
+            const bf16x8 Llo = __builtin_bit_cast(bf16x8, lf[e + 1]);
+            accA[i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Llo, ah, accA[i], 0, 0, 0);
+            accB[i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Llo, bh, accB[i], 0, 0, 0);
+            accA[i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Lhi, al, accA[i], 0, 0, 0);
+            accB[i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Lhi, bl, accB[i], 0, 0, 0);
+            accA[i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Lhi, ah, accA[i], 0, 0, 0);
+            accB[i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Lhi, bh, accB[i], 0, 0, 0);
+          }
+        }
+      }
+      __syncthreads();
+    }
+    uint32_t SA = 0, SB = 0;
+#pragma unroll
+    for (int i = 0; i < TW; ++i) {
+      if (steps[i] > 0) {
+        const bool last = tile[i] == T - 1;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const bool excl = last && r == 15 && g == 1;  // the loss row
+          SA += (accA[i][r] < a.thr && !excl) ? 1u : 0u;
+          SB += (accB[i][r] < a.thr && !excl) ? 1u : 0u;
+        }
+      }
+    }
+    swap32(SA, SB);
+    sh.sS[wave][lane] = SA + SB;
+    if (wave == loss_wave) {
+      float la = 0.0f, lb = 0.0f;
+#pragma unroll
+      for (int i = 0; i < TW; ++i)
+        if (i == loss_slot) {
+          la = accA[i][15];
+          lb = accB[i][15];
+        }
+      uint32_t ua = as_u32(la), ub = as_u32(lb);
+      swap32(ua, ub);
+      sh.sLoss[lane] = as_f32(ub);
+    }
+    __syncthreads();
+    const uint64_t q = grp * 64 + lane;
+    if (wave == 0 && q < a.npaths) {
+      int Sp = 0;
+#pragma unroll
+      for (int v = 0; v < NC; ++v) Sp += (int)sh.sS[v][lane];
+      tally_path(a, t, q, Sp, sh.sLoss[lane], false, sh_hist);
+    }
+  }
+}
+
+template <int NC, int TW, int P>
+__global__ __launch_bounds__(64 * (NC + kProducers)) void k_mfma_spec(PathArgs a) {
+  __shared__ SpecShared<NC, P> sh;
+  __shared__ unsigned sh_hist[kMaxN + 1];
+  const bool want_hist = a.s_hist != nullptr;
+  if (want_hist) {
+    for (int i = threadIdx.x; i <= a.n; i += blockDim.x) sh_hist[i] = 0u;
+  }
+  LaneTally t;
+  zero_tally(t);
+  mfma_spec_groups<NC, TW, P>(a, blockIdx.x, (a.npaths + 63) / 64, gridDim.x, sh, t,
+                              want_hist ? sh_hist : nullptr);
+  flush_tally(a, t);
+  if (want_hist) {
+    __syncthreads();
+    for (int i = threadIdx.x; i <= a.n; i += blockDim.x)
+      if (sh_hist[i]) atomicAdd(&a.s_hist[i], (unsigned long long)sh_hist[i]);
+  }
+}
+
+template <int NC, int TW, int P>
+__global__ __launch_bounds__(64 * (NC + kProducers)) void k_batch_spec(BatchArgs b) {
+  __shared__ SpecShared<NC, P> sh;
+  __shared__ unsigned s_item;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = atomicAdd(b.next, 1u);
+    __syncthreads();
+    const unsigned it = __builtin_amdgcn_readfirstlane(s_item);
+    __syncthreads();
+    if (it >= b.nitems) break;
+    const WorkItem w = load_const(b.items + it);
+    const PathArgs a = load_const(b.qs + w.query);
+    LaneTally t;
+    zero_tally(t);
+    mfma_spec_groups<NC, TW, P>(a, w.first_group, w.first_group + w.ngroups, 1, sh, t, nullptr);
+    flush_tally(a, t);
+  }
+}
+
 __global__ void k_batch_clear(unsigned* next, int count) {
   for (int i = threadIdx.x; i < count; i += blockDim.x) next[i] = 0u;
 }
@@ -302,17 +490,19 @@ hipError_t launch_batch_kernel(BatchKernel k, const BatchArgs& b, int grid, int 
 // four waves with up to 3 tiles each (several workgroups per CU, so one's
 // barrier waits are covered by another's work); eight waves with 2 tiles
 // each at four waves per SIMD for 13-16 tiles (one wave per SIMD more than
-// four waves with 4 tiles); eight waves with 3 tiles each beyond, with the
-// next-step fragment prefetch (-16 % at n = 600; it costs the other shapes a
-// wave per SIMD).
-int big_waves(int T) { return T <= 12 ? 4 : 8; }
+// four waves with 4 tiles); beyond, 12 multiplying waves with 2 tiles each
+// beside 4 generating waves (n = 600: 15.2-15.6 ms against 16.8 ms for eight
+// waves with 3 tiles each and the next-step fragment prefetch).  The split form
+// lost to the shapes above for fewer tiles (8 + 4 waves: -1 % at n = 450, +2 %
+// on the uneven batch; 4 + 4 waves: +11 % at n = 300).
+int big_waves(int T) { return T <= 12 ? 4 : (T <= 16 ? 8 : 12 + kProducers); }
 
 BatchKernel batch_big_kernel(int T) {
   if (T <= 4) return k_batch_paths<1, 4, true>;
   if (T <= 8) return k_batch_paths<2, 4, false>;
   if (T <= 12) return k_batch_paths<3, 4, false>;
   if (T <= 16) return k_batch_paths<2, 8, false>;
-  return T <= 24 ? k_batch_paths<3, 8, true> : nullptr;
+  return T <= 24 ? k_batch_spec<12, 2, 2> : nullptr;
 }
 
 PathKernel mfma_big_kernel(int T) {
@@ -320,7 +510,7 @@ PathKernel mfma_big_kernel(int T) {
   if (T <= 8) return k_mfma_big<2, 4, false>;
   if (T <= 12) return k_mfma_big<3, 4, false>;
   if (T <= 16) return k_mfma_big<2, 8, false>;
-  return T <= 24 ? k_mfma_big<3, 8, true> : nullptr;
+  return T <= 24 ? k_mfma_spec<12, 2, 2> : nullptr;
 }
 
 }  // namespace mcr

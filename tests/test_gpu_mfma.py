@@ -45,9 +45,10 @@ def test_mfma_vs_fp64_same_stream(mcr, orc, n):
 def test_mfma_lds_vs_fp64_same_stream(mcr, orc, n, M):
     """Large-n MFMA kernel (normals staged in LDS), same tolerances.  The sizes
     cover every workgroup shape (T = 1, 2, 5, 7, 11 row tiles with four
-    waves; 15 with eight waves of 2 tiles; 17, 19, 20 with eight of 3 and the
-    fragment prefetch), an odd step count with eight waves (n = 520: S = 33,
-    the last phase half full) and the largest n."""
+    waves; 15 with eight waves of 2 tiles; 17, 19, 20 with twelve multiplying
+    waves of 2 tiles beside four generating waves), an odd step count with
+    two-step phases (n = 520: S = 33, the last phase half full) and the
+    largest n."""
     seed, x = 91, 0.02
     k = max(1, n // 5)
     mu, sigma = factor_like(n, min(20, n), 70 + n, T=max(4000, 2 * n))
