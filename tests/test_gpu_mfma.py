@@ -40,13 +40,14 @@ def test_mfma_vs_fp64_same_stream(mcr, orc, n):
 
 
 @pytest.mark.parametrize("n,M", [(5, 100_000), (50, 100_000), (128, 40_000), (200, 20_000),
-                                 (333, 10_000), (450, 8_000), (520, 6_000), (600, 6_000),
-                                 (608, 5_000)])
+                                 (333, 10_000), (450, 8_000), (511, 6_000), (512, 6_000),
+                                 (520, 6_000), (600, 6_000), (608, 5_000)])
 def test_mfma_lds_vs_fp64_same_stream(mcr, orc, n, M):
     """Large-n MFMA kernel (normals staged in LDS), same tolerances.  The sizes
     cover every workgroup shape (T = 1, 2, 5, 7, 11 row tiles with four
     waves; 15 with eight waves of 2 tiles; 17, 19, 20 with twelve multiplying
-    waves of 2 tiles beside four generating waves), an odd step count with
+    waves of 2 tiles beside four generating waves), the boundary between the
+    last two (n = 511: T = 16; n = 512: T = 17), an odd step count with
     two-step phases (n = 520: S = 33, the last phase half full) and the
     largest n."""
     seed, x = 91, 0.02
