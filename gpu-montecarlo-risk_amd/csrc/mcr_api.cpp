@@ -669,7 +669,7 @@ BatchKernel class_kernel(int n) {
   return n <= kSmallMaxN ? batch_small_kernel(n, T) : batch_big_kernel(T);
 }
 
-int class_block(int n) { return n <= kSmallMaxN ? 256 : 64 * big_waves((n + 1 + 31) / 32); }
+int class_block(int n) { return n <= kSmallMaxN ? 256 : 64 * batch_big_waves((n + 1 + 31) / 32); }
 
 int class_grid(BatchKernel k, int block) {
   int cus = g_ctx.num_cus;

@@ -135,8 +135,9 @@ struct BatchArgs {
 // per workgroup and row-tile slots per wave).
 typedef void (*BatchKernel)(BatchArgs);
 BatchKernel batch_small_kernel(int n, int T);  // n <= 127, 256 threads
-BatchKernel batch_big_kernel(int T);           // T <= 24, 64 * big_waves(T) threads
-int big_waves(int T);                          // waves per workgroup of the LDS MFMA kernel
+BatchKernel batch_big_kernel(int T);           // T <= 24, 64 * batch_big_waves(T) threads
+int big_waves(int T);                          // waves per workgroup, single-query LDS MFMA kernel
+int batch_big_waves(int T);                    // waves per workgroup, batch LDS MFMA kernel
 int resident_blocks(const void* kernel, int block);  // workgroups per CU (occupancy)
 hipError_t launch_batch_clear(unsigned* next, int count, hipStream_t stream);
 hipError_t launch_batch_kernel(BatchKernel k, const BatchArgs& b, int grid, int block,

@@ -6,7 +6,7 @@
 // with the same augmented rows as the small MFMA kernel (assets, +inf pads,
 // loss row 32T-1 = -v, initial -w^T mu) and the same bf16 hi/lo split.
 //
-// A workgroup of NW waves (4, or 8 for 384 <= n < 512) owns 64 paths.  The 64 x 32T
+// A workgroup of NW waves (4; 8 for 13-16 row tiles in batches) owns 64 paths.  The 64 x 32T
 // accumulator block is spread over the waves' VGPRs: row tile t belongs to
 // wave (t/NW even ? t%NW : NW-1-t%NW) (snake order, since tile t needs
 // min(2t+2, S) steps), at most TW tiles per wave; the shape keeps at least two
@@ -15,8 +15,9 @@
 // Philox block (4 normals of one path) and writes its bf16 hi/lo parts
 // straight into a double-buffered LDS tile already in B-operand order, so the
 // next phase is generated while this one is multiplied; factor fragments
-// stream from L2.  For n >= 512 (17-24 row tiles) the roles are split
-// instead: 12 waves only multiply, 4 only generate (k_mfma_spec below).
+// stream from L2.  For n >= 384 (13-24 row tiles; batches from 17 tiles) the
+// roles are split instead: 8 or 12 waves only multiply, 4 only generate
+// (k_mfma_spec below).
 #include "mcr_kern_common.h"
 
 namespace mcr {
@@ -293,6 +294,10 @@ __global__ __launch_bounds__(64 * NW, big_min_waves(TW, NW)) void k_batch_paths(
 // ---------------------------------------------------------------------------
 constexpr int kProducers = 4;
 
+// Word 3 of a raw buffer resource on gfx9-family parts (stride 0, untyped
+// dword loads).
+constexpr int kBufferRsrcWord3 = 0x00020000;
+
 template <int NC, int P>
 struct SpecShared {
   ZTile<4 * P> zt;  // P steps per phase, double-buffered
@@ -340,7 +345,14 @@ __device__ __forceinline__ void mfma_spec_groups(const PathArgs& a, uint64_t g0,
     return;
   }
 
-  const uint4* __restrict__ lf = a.lfrag + lane * 2;
+  // Factor fragments through a buffer resource: fragment (tile, s) of this
+  // lane is at byte (tile S + s) 2048 + 32 lane (+16 for lo); the per-step
+  // part is a scalar offset, so no VGPR is rewritten while a load is pending
+  // (a per-lane 64-bit address would be, and each step would wait for the
+  // previous tile's loads before issuing the next tile's).
+  const __amdgpu_buffer_rsrc_t lrs = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<uint4*>(a.lfrag), (short)0, T * S * 2048, kBufferRsrcWord3);
+  const int voff = lane * 32;
   const float4* __restrict__ rinit = (const float4*)a.rowinit;
   int tile[TW], steps[TW];
 #pragma unroll
@@ -373,6 +385,18 @@ __device__ __forceinline__ void mfma_spec_groups(const PathArgs& a, uint64_t g0,
       for (int q = 0; q < P; ++q) {
         const int s = ph * P + q;
         if (s >= S) break;
+        // every active tile's factor fragments are requested before the
+        // first MFMA of the step, so the loads of all tiles are in flight
+        // together instead of one tile's at a time
+        uint4 fh[TW], fl[TW];
+#pragma unroll
+        for (int i = 0; i < TW; ++i) {
+          if (s < steps[i]) {
+            const int e = (tile[i] * S + s) * 2048;
+            fh[i] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(lrs, voff, e, 0));
+            fl[i] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(lrs, voff + 16, e, 0));
+          }
+        }
         const bf16x8 ah = __builtin_bit_cast(bf16x8, sh.zt.v[cb][q][0][0][lane]);
         const bf16x8 al = __builtin_bit_cast(bf16x8, sh.zt.v[cb][q][0][1][lane]);
         const bf16x8 bh = __builtin_bit_cast(bf16x8, sh.zt.v[cb][q][1][0][lane]);
@@ -380,9 +404,8 @@ __device__ __forceinline__ void mfma_spec_groups(const PathArgs& a, uint64_t g0,
 #pragma unroll
         for (int i = 0; i < TW; ++i) {
           if (s < steps[i]) {
-            const size_t e = (size_t)(tile[i] * S + s) * 128;
-            const bf16x8 Lhi = __builtin_bit_cast(bf16x8, lf[e]);
-            const bf16x8 Llo = __builtin_bit_cast(bf16x8, lf[e + 1]);
+            const bf16x8 Lhi = __builtin_bit_cast(bf16x8, fh[i]);
+            const bf16x8 Llo = __builtin_bit_cast(bf16x8, fl[i]);
             accA[i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Llo, ah, accA[i], 0, 0, 0);
             accB[i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Llo, bh, accB[i], 0, 0, 0);
             accA[i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Lhi, al, accA[i], 0, 0, 0);
@@ -486,16 +509,20 @@ hipError_t launch_batch_kernel(BatchKernel k, const BatchArgs& b, int grid, int 
   return hipGetLastError();
 }
 
-// Shape of the LDS MFMA kernel for T row tiles (measured best per range):
-// four waves with up to 3 tiles each (several workgroups per CU, so one's
-// barrier waits are covered by another's work); eight waves with 2 tiles
-// each at four waves per SIMD for 13-16 tiles (one wave per SIMD more than
-// four waves with 4 tiles); beyond, 12 multiplying waves with 2 tiles each
-// beside 4 generating waves (n = 600: 15.2-15.6 ms against 16.8 ms for eight
-// waves with 3 tiles each and the next-step fragment prefetch).  The split form
-// lost to the shapes above for fewer tiles (8 + 4 waves: -1 % at n = 450, +2 %
-// on the uneven batch; 4 + 4 waves: +11 % at n = 300).
-int big_waves(int T) { return T <= 12 ? 4 : (T <= 16 ? 8 : 12 + kProducers); }
+// Shapes of the LDS MFMA kernels for T row tiles (measured best per range).
+// Four waves with up to 3 tiles each for T <= 12 (several workgroups per CU,
+// so one's barrier waits are covered by another's work); beyond, single
+// queries run the split form: 8 (T <= 16) or 12 multiplying waves of 2 tiles
+// beside 4 generating waves (n = 450: 6.0 ms against 6.4 ms for eight waves of
+// 2 tiles; n = 600: 14.6 ms against 16.8 ms for eight waves of 3 tiles with the
+// next-step fragment prefetch).  Batches keep eight waves of 2 tiles at four
+// waves per SIMD for 13-16 tiles: a persistent workgroup there runs many short
+// work items, and two per CU cover each other's item changes (the 8 + 4 split
+// form, one workgroup per CU, made the uneven batch 2 % slower).  The split
+// form loses below 13 tiles (12 + 4 waves of 1 tile: +3 % at n = 300; 8 + 4 of
+// 2: +5 %; 4 + 4 of 3: +9 %).
+int big_waves(int T) { return T <= 12 ? 4 : (T <= 16 ? 8 : 12) + kProducers; }
+int batch_big_waves(int T) { return T <= 12 ? 4 : (T <= 16 ? 8 : 12 + kProducers); }
 
 BatchKernel batch_big_kernel(int T) {
   if (T <= 4) return k_batch_paths<1, 4, true>;
@@ -509,7 +536,7 @@ PathKernel mfma_big_kernel(int T) {
   if (T <= 4) return k_mfma_big<1, 4, true>;
   if (T <= 8) return k_mfma_big<2, 4, false>;
   if (T <= 12) return k_mfma_big<3, 4, false>;
-  if (T <= 16) return k_mfma_big<2, 8, false>;
+  if (T <= 16) return k_mfma_spec<8, 2, 2>;
   return T <= 24 ? k_mfma_spec<12, 2, 2> : nullptr;
 }
 
