@@ -53,6 +53,26 @@ __device__ __forceinline__ void gen_step(ZTile<NW>* zt, int buf, int sp, int s, 
   *lo = make_uint2(l0, l1);
 }
 
+// Factor fragments are read through a buffer resource: fragment (tile, s) of
+// a lane is at byte (tile S + s) 2048 + 32 lane (+16 for the lo part).  The
+// per-step part of the address is then a scalar offset, so no VGPR is
+// rewritten while a load is pending; with a per-lane 64-bit address the
+// compiler reuses the registers of a pending load for the next address, and
+// each step waits for the previous tile's loads before issuing the next.
+constexpr int kBufferRsrcWord3 = 0x00020000;  // raw buffer, gfx9 family
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t frag_rsrc(const PathArgs& a) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<uint4*>(a.lfrag), (short)0,
+                                           a.T * a.S * 2048, kBufferRsrcWord3);
+}
+
+// part 0: hi, 1: lo
+__device__ __forceinline__ uint4 load_frag(__amdgpu_buffer_rsrc_t rs, int voff, int tile, int S,
+                                           int s, int part) {
+  return __builtin_bit_cast(
+      uint4, __builtin_amdgcn_raw_buffer_load_b128(rs, voff + 16 * part, (tile * S + s) * 2048, 0));
+}
+
 template <int NW>
 struct BigShared {
   ZTile<NW> zt;
@@ -80,7 +100,8 @@ __device__ __forceinline__ void mfma_lds_groups(const PathArgs& a, uint64_t g0, 
   const int nb = (n + 3) >> 2;
   const int nph = (S + P - 1) / P;
   const Key2 key = key_of(a.seed);
-  const uint4* __restrict__ lf = a.lfrag + lane * 2;
+  const __amdgpu_buffer_rsrc_t lrs = frag_rsrc(a);
+  const int voff = lane * 32;
   const float4* __restrict__ rinit = (const float4*)a.rowinit;
 
   int tile[TW], steps[TW];
@@ -120,15 +141,26 @@ __device__ __forceinline__ void mfma_lds_groups(const PathArgs& a, uint64_t g0, 
     }
     // One K step s (slot q of the phase in LDS buffer cb) with this step's
     // factor fragments ch/cl; with PF the next step's are loaded into nh/nl
-    // before the MFMAs are issued.
+    // before the MFMAs are issued, without it every active tile's fragments
+    // of this step are (eh/el), so the tiles' loads are in flight together
+    // (the registers this takes spill a few values outside the step loop).
     auto k_step = [&](int s, int q, int cb, const uint4 (&ch)[TW], const uint4 (&cl)[TW],
                       uint4 (&nh)[TW], uint4 (&nl)[TW]) {
 #pragma unroll
       for (int i = 0; i < TW; ++i) {
         if (PF && s + 1 < steps[i]) {
-          const size_t e = (size_t)(tile[i] * S + s + 1) * 128;
-          nh[i] = lf[e];
-          nl[i] = lf[e + 1];
+          nh[i] = load_frag(lrs, voff, tile[i], S, s + 1, 0);
+          nl[i] = load_frag(lrs, voff, tile[i], S, s + 1, 1);
+        }
+      }
+      uint4 eh[TW], el[TW];
+      if (!PF) {
+#pragma unroll
+        for (int i = 0; i < TW; ++i) {
+          if (s < steps[i]) {
+            eh[i] = load_frag(lrs, voff, tile[i], S, s, 0);
+            el[i] = load_frag(lrs, voff, tile[i], S, s, 1);
+          }
         }
       }
       const bf16x8 ah = __builtin_bit_cast(bf16x8, sh.zt.v[cb][q][0][0][lane]);
@@ -140,9 +172,8 @@ __device__ __forceinline__ void mfma_lds_groups(const PathArgs& a, uint64_t g0, 
         if (s < steps[i]) {
           uint4 h = ch[i], l = cl[i];
           if (!PF) {
-            const size_t e = (size_t)(tile[i] * S + s) * 128;
-            h = lf[e];
-            l = lf[e + 1];
+            h = eh[i];
+            l = el[i];
           }
           const bf16x8 Lhi = __builtin_bit_cast(bf16x8, h);
           const bf16x8 Llo = __builtin_bit_cast(bf16x8, l);
@@ -162,9 +193,8 @@ __device__ __forceinline__ void mfma_lds_groups(const PathArgs& a, uint64_t g0, 
 #pragma unroll
     for (int i = 0; i < TW; ++i) {
       if (PF && steps[i] > 0) {
-        const size_t e = (size_t)(tile[i] * S) * 128;
-        LAh[i] = lf[e];
-        LAl[i] = lf[e + 1];
+        LAh[i] = load_frag(lrs, voff, tile[i], S, 0, 0);
+        LAl[i] = load_frag(lrs, voff, tile[i], S, 0, 1);
       }
     }
     for (int ph = 0; ph < nph; ++ph) {
@@ -294,10 +324,6 @@ __global__ __launch_bounds__(64 * NW, big_min_waves(TW, NW)) void k_batch_paths(
 // ---------------------------------------------------------------------------
 constexpr int kProducers = 4;
 
-// Word 3 of a raw buffer resource on gfx9-family parts (stride 0, untyped
-// dword loads).
-constexpr int kBufferRsrcWord3 = 0x00020000;
-
 template <int NC, int P>
 struct SpecShared {
   ZTile<4 * P> zt;  // P steps per phase, double-buffered
@@ -345,13 +371,7 @@ __device__ __forceinline__ void mfma_spec_groups(const PathArgs& a, uint64_t g0,
     return;
   }
 
-  // Factor fragments through a buffer resource: fragment (tile, s) of this
-  // lane is at byte (tile S + s) 2048 + 32 lane (+16 for lo); the per-step
-  // part is a scalar offset, so no VGPR is rewritten while a load is pending
-  // (a per-lane 64-bit address would be, and each step would wait for the
-  // previous tile's loads before issuing the next tile's).
-  const __amdgpu_buffer_rsrc_t lrs = __builtin_amdgcn_make_buffer_rsrc(
-      const_cast<uint4*>(a.lfrag), (short)0, T * S * 2048, kBufferRsrcWord3);
+  const __amdgpu_buffer_rsrc_t lrs = frag_rsrc(a);
   const int voff = lane * 32;
   const float4* __restrict__ rinit = (const float4*)a.rowinit;
   int tile[TW], steps[TW];
@@ -392,9 +412,8 @@ __device__ __forceinline__ void mfma_spec_groups(const PathArgs& a, uint64_t g0,
 #pragma unroll
         for (int i = 0; i < TW; ++i) {
           if (s < steps[i]) {
-            const int e = (tile[i] * S + s) * 2048;
-            fh[i] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(lrs, voff, e, 0));
-            fl[i] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(lrs, voff + 16, e, 0));
+            fh[i] = load_frag(lrs, voff, tile[i], S, s, 0);
+            fl[i] = load_frag(lrs, voff, tile[i], S, s, 1);
           }
         }
         const bf16x8 ah = __builtin_bit_cast(bf16x8, sh.zt.v[cb][q][0][0][lane]);
