@@ -61,9 +61,16 @@ __device__ __forceinline__ void gen_step(ZTile<NW>* zt, int buf, int sp, int s, 
 // each step waits for the previous tile's loads before issuing the next.
 constexpr int kBufferRsrcWord3 = 0x00020000;  // raw buffer, gfx9 family
 
+// The base and size go through readfirstlane: where the query record came
+// from memory the compiler cannot prove them uniform, and a descriptor it
+// takes for divergent is used through a loop over lanes at every load.
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t frag_rsrc(const PathArgs& a) {
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<uint4*>(a.lfrag), (short)0,
-                                           a.T * a.S * 2048, kBufferRsrcWord3);
+  const uint64_t p = (uint64_t)a.lfrag;
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)p);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(p >> 32));
+  const int bytes = __builtin_amdgcn_readfirstlane(a.T * a.S * 2048);
+  return __builtin_amdgcn_make_buffer_rsrc((void*)(((uint64_t)hi << 32) | lo), (short)0, bytes,
+                                           kBufferRsrcWord3);
 }
 
 // part 0: hi, 1: lo
