@@ -1181,7 +1181,7 @@ int mcr_prepare(const mcr_query* q, mcr_model_export* out) {
     std::vector<float> l((size_t)st.plan.r8 * st.plan.r8);
     HIP_TRY(hipMemcpy(l.data(), st.d.L32, l.size() * sizeof(float), hipMemcpyDeviceToHost));
     for (int i = 0; i < n; ++i)
-      for (int j = 0; j < n; ++j) out->L32[(size_t)i * n + j] = l[(size_t)i * st.plan.r8 + j];
+      for (int j = 0; j < n; ++j) out->L32[(size_t)i * n + j] = l[l32_offset(st.plan.r8, i, j)];
   }
   if (out->mu32) HIP_TRY(hipMemcpy(out->mu32, st.d.mu32, n * sizeof(float), hipMemcpyDeviceToHost));
   if (out->v32) HIP_TRY(hipMemcpy(out->v32, st.d.v32, n * sizeof(float), hipMemcpyDeviceToHost));

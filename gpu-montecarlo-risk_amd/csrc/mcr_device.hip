@@ -342,10 +342,10 @@ __device__ void prep_body(const PrepArgs& a, size_t gid, size_t gsz) {
       const int i = (int)(e / n), j = (int)(e % n);
       if (j <= i) a.colpack[colpack_offset(n, j) + i - j] = (float)L[e];
     }
-    // tile kernel: zero-padded r8 x r8 row-major
+    // tile kernel: zero-padded r8 x r8 in 8 x 4 blocks (l32_offset)
     for (size_t e = gid; e < (size_t)r8 * r8; e += gsz) {
       const int i = (int)(e / r8), j = (int)(e % r8);
-      a.L32[e] = (i < n && j <= i) ? (float)L[(size_t)i * n + j] : 0.0f;
+      a.L32[l32_offset(r8, i, j)] = (i < n && j <= i) ? (float)L[(size_t)i * n + j] : 0.0f;
     }
   }
   // MFMA: augmented rows (assets, zero pads, loss row 32T-1 = -v), hi/lo bf16
@@ -498,15 +498,19 @@ __global__ __launch_bounds__(64 * NW) void k_valu_tile(PathArgs a) {
       float acc[kTileRows];
 #pragma unroll
       for (int r = 0; r < kTileRows; ++r) acc[r] = (i0 + r < n) ? mu[i0 + r] : 0.0f;
-      for (int jb = 0; jb < i0 / 4; ++jb) {
+      // Row block rb's 8 x 4 factor blocks (32 floats each, column block jb at
+      // Lb + 32 jb): one K step is two 64-byte scalar loads and one wait.
+      MCR_CONSTP(float) Lb = L + l32_offset(ldl, i0, 0);
+      const int J = i0 / 4;
+      for (int jb = 0; jb < J; ++jb) {
         const float4 z4 = zs[jb * 64 + lane];
+        MCR_CONSTP(float) lv = Lb + 32 * jb;
 #pragma unroll
         for (int r = 0; r < kTileRows; ++r) {
-          MCR_CONSTP(float) Lr = L + (size_t)(i0 + r) * ldl + 4 * jb;
-          acc[r] = __builtin_fmaf(Lr[0], z4.x, acc[r]);
-          acc[r] = __builtin_fmaf(Lr[1], z4.y, acc[r]);
-          acc[r] = __builtin_fmaf(Lr[2], z4.z, acc[r]);
-          acc[r] = __builtin_fmaf(Lr[3], z4.w, acc[r]);
+          acc[r] = __builtin_fmaf(lv[4 * r + 0], z4.x, acc[r]);
+          acc[r] = __builtin_fmaf(lv[4 * r + 1], z4.y, acc[r]);
+          acc[r] = __builtin_fmaf(lv[4 * r + 2], z4.z, acc[r]);
+          acc[r] = __builtin_fmaf(lv[4 * r + 3], z4.w, acc[r]);
         }
       }
       // diagonal 8x8 block: row i0+r uses columns i0 .. i0+r
@@ -523,9 +527,9 @@ __global__ __launch_bounds__(64 * NW) void k_valu_tile(PathArgs a) {
       }
 #pragma unroll
       for (int r = 0; r < kTileRows; ++r) {
-        MCR_CONSTP(float) Lr = L + (size_t)(i0 + r) * ldl + i0;
 #pragma unroll
-        for (int c = 0; c <= r; ++c) acc[r] = __builtin_fmaf(Lr[c], zd[c], acc[r]);
+        for (int c = 0; c <= r; ++c)
+          acc[r] = __builtin_fmaf(Lb[32 * (J + (c >> 2)) + 4 * r + (c & 3)], zd[c], acc[r]);
       }
 #pragma unroll
       for (int r = 0; r < kTileRows; ++r) {

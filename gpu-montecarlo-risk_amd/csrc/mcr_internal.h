@@ -56,7 +56,7 @@ struct PathArgs {
   float thr;        // (float)(-x)
   float thr_lo, thr_hi;  // thr -/+ 1 ulp (near-threshold accounting)
   const float* colpack;  // register kernel: column j = L[j..n-1][j], v[j]
-  const float* L32;      // tile kernel: row-major n x n fp32 factor
+  const float* L32;      // tile kernel: fp32 factor in 8 x 4 blocks (l32_offset)
   const float* mu32;     // [n]
   const float* v32;      // [n] L^T w
   const uint4* lfrag;    // MFMA: [T][S][64 lanes][hi, lo] 8 x bf16 each
@@ -147,6 +147,13 @@ hipError_t launch_normals(uint64_t seed, uint64_t path0, uint64_t npaths, int n,
                           hipStream_t stream);
 
 __host__ __device__ inline int colpack_offset(int n, int j) { return j * (n + 1) - (j * (j - 1)) / 2; }
+// Tile kernel factor: zero-padded r8 x r8 (r8 = n rounded up to kTileRows),
+// stored as 8-row x 4-column blocks, row block major: block (i / 8, j / 4) is
+// 32 contiguous floats (row i % 8, column j % 4), so one K step of a row
+// block is two 64-byte scalar loads.
+__host__ __device__ inline size_t l32_offset(int r8, int i, int j) {
+  return ((((size_t)(i >> 3) * (size_t)(r8 >> 2) + (size_t)(j >> 2)) * 8 + (i & 7)) << 2) + (j & 3);
+}
 __host__ __device__ inline size_t colpack_size(int n) { return (size_t)colpack_offset(n, n); }
 
 }  // namespace mcr
