@@ -544,7 +544,8 @@ hipError_t launch_batch_kernel(BatchKernel k, const BatchArgs& b, int grid, int 
 // next-step fragment prefetch).  Batches keep eight waves of 2 tiles at four
 // waves per SIMD for 13-16 tiles: a persistent workgroup there runs many short
 // work items, and two per CU cover each other's item changes (the 8 + 4 split
-// form, one workgroup per CU, made the uneven batch 2 % slower).  The split
+// form, one workgroup per CU, made the uneven batch 3 % slower; 12 + 4 waves
+// of 1 tile for 9-12 tiles as well, 10 %).  The split
 // form loses below 13 tiles (12 + 4 waves of 1 tile: +3 % at n = 300; 8 + 4 of
 // 2: +5 %; 4 + 4 of 3: +9 %).
 int big_waves(int T) { return T <= 12 ? 4 : (T <= 16 ? 8 : 12) + kProducers; }
