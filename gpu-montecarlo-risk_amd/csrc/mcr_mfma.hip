@@ -59,8 +59,13 @@ __device__ __forceinline__ void mfma_wave_groups(const PathArgs& a, uint64_t g0,
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
           const int b = 4 * s + 2 * h + u;
-          if (b < NB) {
+          if (b < NB - 1 || (b == NB - 1 && 4 * b + 2 < a.n)) {
             normals4(key, path, (uint32_t)b, &z[4 * u]);
+          } else if (b == NB - 1) {
+            // the last block's second pair (normals 4b+2, 4b+3 >= n) only
+            // meets zero factor columns: skipped, as zeros
+            normals2(key, path, (uint32_t)b, &z[4 * u]);
+            z[4 * u + 2] = z[4 * u + 3] = 0.0f;
           } else {
             z[4 * u] = z[4 * u + 1] = z[4 * u + 2] = z[4 * u + 3] = 0.0f;
           }

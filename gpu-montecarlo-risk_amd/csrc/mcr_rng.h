@@ -178,4 +178,11 @@ MCR_HD void normals4(Key2 key, uint64_t path, uint32_t blk, float z[4]) {
   box_muller(x.z, x.w, &z[2], &z[3]);
 }
 
+// normals4's z[0], z[1] alone (the same bits): for a path's last block when
+// n leaves its second pair unused.
+MCR_HD void normals2(Key2 key, uint64_t path, uint32_t blk, float z[2]) {
+  const Ctr4 x = philox4x32_10(Ctr4{blk, 0u, (uint32_t)path, (uint32_t)(path >> 32)}, key.k0, key.k1);
+  box_muller(x.x, x.y, &z[0], &z[1]);
+}
+
 }  // namespace mcr
