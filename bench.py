@@ -23,7 +23,7 @@ PEAK_BF16_TFLOPS = 2500.0   # MI355X published dense bf16 MFMA
 # HBM bytes per launch of the workload's path kernel, from two rocprofv3 --pmc
 # runs of this bench (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 KiB, the factors
 # read off a 256 MiB torch.neg calibration kernel in the same runs).
-TRAFFIC_BYTES = {"mfma": (174.00 * 2.000 + 1410.81) * 1024}
+TRAFFIC_BYTES = {"mfma": (178.50 * 2.000 + 1410.66) * 1024}
 
 WORKLOAD = {"name": "fused_n50_100M", "n": 50, "factors": 5, "days": 2000, "paths": 100_000_000,
             "k": 10, "x": 0.02, "alpha": 0.99, "data_seed": 1002, "path_seed": 20251202}
