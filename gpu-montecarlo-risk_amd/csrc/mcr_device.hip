@@ -499,9 +499,13 @@ __global__ __launch_bounds__(64 * NW) void k_valu_tile(PathArgs a) {
 #pragma unroll
       for (int r = 0; r < kTileRows; ++r) acc[r] = (i0 + r < n) ? mu[i0 + r] : 0.0f;
       // Row block rb's 8 x 4 factor blocks (32 floats each, column block jb at
-      // Lb + 32 jb): one K step is two 64-byte scalar loads and one wait.
+      // Lb + 32 jb): one K step is two 64-byte scalar loads.
       MCR_CONSTP(float) Lb = L + l32_offset(ldl, i0, 0);
       const int J = i0 / 4;
+      // Two K steps per wait where the row blocks are long (NW >= 8: n >~ 150;
+      // n = 400 -8 %); with four waves the unrolled loop measured slower.
+      constexpr int kStepUnroll = NW >= 8 ? 2 : 1;
+#pragma unroll kStepUnroll
       for (int jb = 0; jb < J; ++jb) {
         const float4 z4 = zs[jb * 64 + lane];
         MCR_CONSTP(float) lv = Lb + 32 * jb;
