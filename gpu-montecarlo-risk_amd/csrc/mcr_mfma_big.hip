@@ -16,8 +16,8 @@
 // straight into a double-buffered LDS tile already in B-operand order, so the
 // next phase is generated while this one is multiplied; factor fragments
 // stream from L2.  For n >= 384 (13-24 row tiles; batches from 17 tiles) the
-// roles are split instead: 8 or 12 waves only multiply, 4 only generate
-// (k_mfma_spec below).
+// roles are split instead: 8 waves (12 in batches) only multiply, 4 only
+// generate (k_mfma_spec below).
 #include "mcr_kern_common.h"
 
 namespace mcr {
@@ -538,17 +538,18 @@ hipError_t launch_batch_kernel(BatchKernel k, const BatchArgs& b, int grid, int 
 // Shapes of the LDS MFMA kernels for T row tiles (measured best per range).
 // Four waves with up to 3 tiles each for T <= 12 (several workgroups per CU,
 // so one's barrier waits are covered by another's work); beyond, single
-// queries run the split form: 8 (T <= 16) or 12 multiplying waves of 2 tiles
+// queries run the split form: 8 multiplying waves of 2 (T <= 16) or 3 tiles
 // beside 4 generating waves (n = 450: 6.0 ms against 6.4 ms for eight waves of
-// 2 tiles; n = 600: 14.6 ms against 16.8 ms for eight waves of 3 tiles with the
-// next-step fragment prefetch).  Batches keep eight waves of 2 tiles at four
-// waves per SIMD for 13-16 tiles: a persistent workgroup there runs many short
-// work items, and two per CU cover each other's item changes (the 8 + 4 split
-// form, one workgroup per CU, made the uneven batch 3 % slower; 12 + 4 waves
-// of 1 tile for 9-12 tiles as well, 10 %).  The split
-// form loses below 13 tiles (12 + 4 waves of 1 tile: +3 % at n = 300; 8 + 4 of
-// 2: +5 %; 4 + 4 of 3: +9 %).
-int big_waves(int T) { return T <= 12 ? 4 : (T <= 16 ? 8 : 12) + kProducers; }
+// 2 tiles; n = 600: 14.85 ms against 15.0 ms for 12 multiplying waves of 2
+// tiles and 16.8 ms for eight waves of 3 tiles that all generate).  Batches
+// keep eight waves of 2 tiles at four waves per SIMD for 13-16 tiles (a
+// persistent workgroup there runs many short work items, and two per CU cover
+// each other's item changes; the 8 + 4 split form, one workgroup per CU, made
+// the uneven batch 3 % slower, and 12 + 4 waves of 1 tile for 9-12 tiles as
+// well 10 %), and 12 multiplying waves of 2 tiles beyond.  The split form
+// loses below 13 tiles (12 + 4 waves of 1 tile: +3 % at n = 300; 8 + 4 of 2:
+// +5 %; 4 + 4 of 3: +9 %).
+int big_waves(int T) { return T <= 12 ? 4 : 8 + kProducers; }
 int batch_big_waves(int T) { return T <= 12 ? 4 : (T <= 16 ? 8 : 12 + kProducers); }
 
 BatchKernel batch_big_kernel(int T) {
@@ -564,7 +565,7 @@ PathKernel mfma_big_kernel(int T) {
   if (T <= 8) return k_mfma_big<2, 4, false>;
   if (T <= 12) return k_mfma_big<3, 4, false>;
   if (T <= 16) return k_mfma_spec<8, 2, 2>;
-  return T <= 24 ? k_mfma_spec<12, 2, 2> : nullptr;
+  return T <= 24 ? k_mfma_spec<8, 3, 2> : nullptr;
 }
 
 }  // namespace mcr

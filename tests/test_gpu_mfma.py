@@ -46,8 +46,8 @@ def test_mfma_lds_vs_fp64_same_stream(mcr, orc, n, M):
     """Large-n MFMA kernel (normals staged in LDS), same tolerances.  The sizes
     cover every single-query workgroup shape (T = 1, 2, 5, 7, 11 row tiles
     with four waves; 15, 16 with eight multiplying waves of 2 tiles beside
-    four generating waves; 17, 19, 20 with twelve), the boundary between the
-    last two (n = 511: T = 16; n = 512: T = 17), an odd step count with
+    four generating waves; 17, 19, 20 with eight of 3 tiles), the boundary
+    between the last two (n = 511: T = 16; n = 512: T = 17), an odd step count with
     two-step phases (n = 520: S = 33, the last phase half full) and the
     largest n.  The batch shapes are covered by the batch tests."""
     seed, x = 91, 0.02
