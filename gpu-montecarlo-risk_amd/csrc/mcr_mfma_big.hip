@@ -16,8 +16,8 @@
 // straight into a double-buffered LDS tile already in B-operand order, so the
 // next phase is generated while this one is multiplied; factor fragments
 // stream from L2.  For n >= 384 (13-24 row tiles; batches from 17 tiles) the
-// roles are split instead: 8 waves (12 in batches) only multiply, 4 only
-// generate (k_mfma_spec below).
+// roles are split instead: 8 waves only multiply, 4 only generate
+// (k_mfma_spec below).
 #include "mcr_kern_common.h"
 
 namespace mcr {
@@ -546,18 +546,18 @@ hipError_t launch_batch_kernel(BatchKernel k, const BatchArgs& b, int grid, int 
 // persistent workgroup there runs many short work items, and two per CU cover
 // each other's item changes; the 8 + 4 split form, one workgroup per CU, made
 // the uneven batch 3 % slower, and 12 + 4 waves of 1 tile for 9-12 tiles as
-// well 10 %), and 12 multiplying waves of 2 tiles beyond.  The split form
+// well 10 %), and the single-query shape beyond.  The split form
 // loses below 13 tiles (12 + 4 waves of 1 tile: +3 % at n = 300; 8 + 4 of 2:
 // +5 %; 4 + 4 of 3: +9 %).
 int big_waves(int T) { return T <= 12 ? 4 : 8 + kProducers; }
-int batch_big_waves(int T) { return T <= 12 ? 4 : (T <= 16 ? 8 : 12 + kProducers); }
+int batch_big_waves(int T) { return T <= 12 ? 4 : (T <= 16 ? 8 : 8 + kProducers); }
 
 BatchKernel batch_big_kernel(int T) {
   if (T <= 4) return k_batch_paths<1, 4, true>;
   if (T <= 8) return k_batch_paths<2, 4, false>;
   if (T <= 12) return k_batch_paths<3, 4, false>;
   if (T <= 16) return k_batch_paths<2, 8, false>;
-  return T <= 24 ? k_batch_spec<12, 2, 2> : nullptr;
+  return T <= 24 ? k_batch_spec<8, 3, 2> : nullptr;
 }
 
 PathKernel mfma_big_kernel(int T) {
