@@ -1,11 +1,14 @@
 """Benchmark of the Monte Carlo crash-risk path on one MI355X.
 
-Workload (one step = one full risk query, factor included): n = 50 assets from
-a seeded 5-factor model (T = 2000 days), 100,000,000 Philox paths, crash
-threshold -2%, k = 10, 99% VaR of the equal-weight portfolio.  Prints one JSON
-line: whole-job paths/s, the path kernel's roofline figures (HIP-event timing
-of the kernel on the library's stream) and a CPU baseline timed on a bounded
-sample of the same workload (the C oracle, serial and OpenMP, host cores).
+Headline workload (one step = one full risk query, factor included): n = 50
+assets from a seeded 5-factor model (T = 2000 days), 100,000,000 Philox paths,
+crash threshold -2 %, k = 10, 99 % VaR of the equal-weight portfolio.  Prints
+one JSON line: whole-job paths/s, the path kernel's roofline figures (HIP-event
+timing of the kernel on the library's stream), a CPU baseline timed on the
+host cores of the GPU's share (the C oracle, serial and OpenMP, on the same
+stream, with the exceedance counts compared at the full 1e8 paths), and the
+other workloads as legs: n = 20 on the serial C path, n = 600 fp32 vs bf16x3,
+256 uneven portfolios batched, and the serving loop at fixed query rates.
 """
 from __future__ import annotations
 
@@ -20,20 +23,15 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP32_TFLOPS = 157.3    # MI355X published fp32
 PEAK_BF16_TFLOPS = 2500.0   # MI355X published dense bf16 MFMA
-# HBM bytes per launch of the workload's path kernel, from two rocprofv3 --pmc
+# HBM bytes per launch of the headline path kernel, from two rocprofv3 --pmc
 # runs of this bench (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 KiB, the factors
 # read off a 256 MiB torch.neg calibration kernel in the same runs).
 TRAFFIC_BYTES = {"mfma": (178.50 * 2.000 + 1410.66) * 1024}
-
-WORKLOAD = {"name": "fused_n50_100M", "n": 50, "factors": 5, "days": 2000, "paths": 100_000_000,
-            "k": 10, "x": 0.02, "alpha": 0.99, "data_seed": 1002, "path_seed": 20251202}
-BATCH = {"name": "batch_256_uneven", "count": 256, "n_range": (5, 600), "paths_range": (1e5, 1e7),
-         "x_range": (0.01, 0.04), "alpha": 0.99, "seed": 1004}
-SERVE = {"name": "serve_poisson_200qps", "qps": 200.0, "seconds": 5.0, "reuse": 0.3,
-         "n_range": (5, 600), "paths_range": (1e5, 1e7), "x_range": (0.01, 0.04), "alpha": 0.99,
-         "seed": 1005}
-LARGE = {"name": "large_n600_10M", "n": 600, "factors": 20, "days": 2500, "paths": 10_000_000,
-         "k": 10, "x": 0.02, "alphas": (0.99, 0.999), "data_seed": 1003, "path_seed": 20251203}
+# VALU figures of the headline path kernel from rocprofv3 --pmc runs of this
+# bench (SQ_ACTIVE_INST_VALU, SQ_VALU_MFMA_BUSY_CYCLES, SQ_INSTS_VALU over
+# GRBM_GUI_ACTIVE): the unit that binds it is the VALU issue of normal
+# generation, not the matrix cores.
+VALU_PROFILE = {"mfma": None}
 
 
 def parse():
@@ -43,218 +41,200 @@ def parse():
     p.add_argument("--warmup", type=int, default=8)
     p.add_argument("--precision", default="auto", choices=["auto", "fp32", "bf16x3"])
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    p.add_argument("--cpu-seconds", type=float, default=12.0)
+    p.add_argument("--cpu-serial-paths", type=int, default=10_000_000,
+                   help="paths of the serial (1-core) CPU sample")
+    p.add_argument("--no-plain", action="store_true", help="skip the n=20 serial-C leg")
     p.add_argument("--no-large", action="store_true", help="skip the n=600 leg")
     p.add_argument("--no-batch", action="store_true", help="skip the 256-portfolio leg")
     p.add_argument("--no-serve", action="store_true", help="skip the serving leg")
-    p.add_argument("--serve-seconds", type=float, default=SERVE["seconds"])
+    p.add_argument("--serve-queries", type=int, default=5000,
+                   help="queries per serving rate (p99 from this many samples)")
+    p.add_argument("--serve-rates", default="200,400,800,1600,3200",
+                   help="offered query rates of the serving sweep (the first is the headline)")
     p.add_argument("--copy-mb", type=int, default=0,
                    help="first run a plain device copy of this many MiB (counter calibration)")
     return p.parse_args()
 
 
-def cpu_baseline(L, mu, x, k, seed, n, budget_s):
-    """C oracle (port of the serial/OpenMP models on the same stream) timed on
-    a bounded sample; returns the OpenMP figure plus the serial one."""
-    import __graft_entry__ as ge
+def cpu_share():
+    """Host threads of this GPU's share: OMP_NUM_THREADS when set (the box sets
+    it to its per-GPU share), else the affinity mask capped at 16."""
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    threads = min(omp, aff) if omp > 0 else min(aff, 16)
+    return max(1, threads), {"os_cpu_count": os.cpu_count(), "affinity_cpus": aff,
+                             "omp_num_threads": omp or None}
+
+
+def cpu_baseline(mcr, orc, W, mu, sigma, gpu_full, serial_paths):
+    """The C oracle (restatement of the serial/OpenMP models on the same Philox
+    stream and fp32 arithmetic) on the host: serial on the first serial_paths
+    paths, OpenMP over every path of the workload; both counts compared with
+    the GPU fp32 kernel's on the same path ranges."""
     import numpy as np
 
-    orc = ge.load_oracle()
-    w = np.full(n, 1.0 / n)
-    model = orc.F32Model.from_f64(L, mu, w, x)
-    threads = min(os.cpu_count() or 1, 16)
-    # serial rate on a short sample, then size the OpenMP sample to the budget
+    n = W["n"]
+    m = mcr.prepare(mu, sigma, W["x"])
+    model = orc.F32Model(m["L32"], m["mu32"], m["v32"], m["mp32"], m["thr32"])
+    threads, host = cpu_share()
+    ns = int(min(serial_paths, W["paths"]))
     t0 = time.perf_counter()
-    ns = 20000
-    orc.simulate_f32(model, seed, 0, ns, k, nthreads=1)
-    serial_rate = ns / (time.perf_counter() - t0)
-    npar = int(min(max(serial_rate * threads * budget_s * 0.7, 100000), 50_000_000))
+    ser = orc.simulate_f32(model, W["path_seed"], 0, ns, W["k"], nthreads=1)
+    serial_s = time.perf_counter() - t0
+    gpu_ser = mcr.run(mu, sigma, W["x"], W["k"], ns, W["path_seed"], precision=mcr.PREC_FP32,
+                      near=True)
     t0 = time.perf_counter()
-    orc.simulate_f32(model, seed, 0, npar, k, nthreads=threads)
-    par_rate = npar / (time.perf_counter() - t0)
-    return {"value": par_rate, "unit": "paths/s", "cores": threads, "kind": "port",
-            "sample": f"{npar} paths of the n=50 workload (OpenMP, {threads} threads); "
-                      f"serial 1-core rate {serial_rate:.4g} paths/s on {ns} paths",
-            "serial_value": serial_rate}
+    par = orc.simulate_f32(model, W["path_seed"], 0, W["paths"], W["k"], nthreads=threads)
+    par_s = time.perf_counter() - t0
+    res = {"value": W["paths"] / par_s, "unit": "paths/s", "cores": threads, "kind": "port",
+           "sample": (f"OpenMP over all {W['paths']} paths of the n={n} workload ({threads} "
+                      f"threads, {par_s:.1f} s); serial on its first {ns} paths ({serial_s:.1f} s)"),
+           "host": host,
+           "serial_value": ns / serial_s, "serial_paths": ns,
+           "count_openmp": par["count"], "count_gpu_fp32": gpu_full.count,
+           "near_openmp": par["near"], "near_gpu_fp32": gpu_full.near_threshold,
+           "count_equal": par["count"] == gpu_full.count and par["near"] == gpu_full.near_threshold,
+           "count_serial": ser["count"], "count_gpu_fp32_serial_range": gpu_ser.count,
+           "count_equal_serial": (ser["count"] == gpu_ser.count
+                                  and ser["near"] == gpu_ser.near_threshold),
+           "gpu_vs_openmp": None}
+    print(f"# cpu: serial {res['serial_value']:.4g} paths/s ({ns} paths), OpenMP x{threads} "
+          f"{res['value']:.4g} paths/s; counts: OpenMP {par['count']} (near {par['near']}) vs GPU "
+          f"fp32 {gpu_full.count} (near {gpu_full.near_threshold}), serial {ser['count']} vs "
+          f"{gpu_ser.count}", file=sys.stderr, flush=True)
+    return res
 
 
-def batch_portfolios(mcr, np, count=None):
-    """Config 4: n log-uniform in [5, 600], paths log-uniform in [1e5, 1e7],
-    Dirichlet(1) weights, thresholds U(1%, 4%); Sigma from the seeded factor
-    model with K = clamp(n/30, 1, 20) factors over T = 2n + 200 days."""
-    B = BATCH
-    rng = np.random.default_rng(B["seed"])
-    out = []
-    for i in range(count or B["count"]):
-        n = int(round(np.exp(rng.uniform(np.log(B["n_range"][0]), np.log(B["n_range"][1])))))
-        m = int(round(np.exp(rng.uniform(np.log(B["paths_range"][0]), np.log(B["paths_range"][1])))))
-        K = int(min(20, max(1, n // 30)))
-        mu, sigma = mcr.synth_factor_model(n, K, 2 * n + 200, B["seed"] * 1000 + i)
-        out.append({"mu": mu, "sigma": sigma, "x": float(rng.uniform(*B["x_range"])),
-                    "k": max(1, (n + 9) // 10), "paths": m, "seed": int(rng.integers(0, 2**62)),
-                    "weights": rng.dirichlet(np.ones(n)), "alphas": (B["alpha"],),
-                    "precision": mcr.PREC_BF16X3})
-    return out
+def plain_leg(mcr, orc, wl):
+    """Config 1 (n = 20, 1M paths, k = 5): the serial C path (oracle, 1 core)
+    against the GPU fp32 kernel on the identical stream, counts compared."""
+    import numpy as np
+
+    C = wl.CONFIG1
+    mu, sigma = wl.model_of(mcr, C)
+    runs = [mcr.run(mu, sigma, C["x"], C["k"], C["paths"], C["path_seed"], precision=mcr.PREC_FP32,
+                    near=True) for _ in range(4)]
+    r = runs[-1]
+    bf = [mcr.run(mu, sigma, C["x"], C["k"], C["paths"], C["path_seed"],
+                  precision=mcr.PREC_BF16X3) for _ in range(4)]
+    m = mcr.prepare(mu, sigma, C["x"])
+    model = orc.F32Model(m["L32"], m["mu32"], m["v32"], m["mp32"], m["thr32"])
+    t0 = time.perf_counter()
+    ser = orc.simulate_f32(model, C["path_seed"], 0, C["paths"], C["k"], nthreads=1)
+    serial_s = time.perf_counter() - t0
+    dev_ms = float(np.mean([x.ms_device for x in runs[1:]]))
+    res = {"workload": C["name"], "n_assets": C["n"], "paths": C["paths"], "k": C["k"],
+           "gpu_fp32": {"kernel": r.kernel, "kernel_ms": float(np.mean([x.ms_kernel for x in runs[1:]])),
+                        "device_ms": dev_ms, "paths_per_s": C["paths"] / (dev_ms * 1e-3),
+                        "count": r.count, "near_threshold": r.near_threshold, "p_hat": r.p_hat,
+                        "std_error": r.std_error},
+           "gpu_bf16x3": {"kernel": bf[-1].kernel,
+                          "kernel_ms": float(np.mean([x.ms_kernel for x in bf[1:]])),
+                          "count": bf[-1].count},
+           "cpu_serial": {"cores": 1, "seconds": serial_s, "paths_per_s": C["paths"] / serial_s,
+                          "count": ser["count"], "near_threshold": ser["near"]},
+           "count_equal": ser["count"] == r.count and ser["near"] == r.near_threshold}
+    print(f"# n=20: GPU fp32 {dev_ms:.3f} ms ({r.kernel}) count {r.count} near {r.near_threshold}; "
+          f"serial C {serial_s:.2f} s count {ser['count']} near {ser['near']}", file=sys.stderr,
+          flush=True)
+    return res
 
 
-def batch_leg(mcr, np):
-    """Config 4: one persistent launch (hipGraph) vs one launch per portfolio."""
-    ps = batch_portfolios(mcr, np)
+def same_result(a, b):
+    return (a.count == b.count and a.var == b.var and a.es == b.es
+            and a.tail_sum_fixed == b.tail_sum_fixed)
+
+
+def batch_leg(mcr, np, wl):
+    """Config 4: persistent class launches in one hipGraph vs one query at a
+    time; every batched result must equal the same query run alone."""
+    ps = wl.batch_portfolios(mcr)
     total = sum(p["paths"] for p in ps)
     best, tried = mcr.autotune_batch(ps)
     r0, o0 = mcr.run_batch(ps, use_graph=True)  # captures
-    times = []
+    dev, wall = [], []
     for _ in range(3):
         r1, o1 = mcr.run_batch(ps, use_graph=True)
-        times.append(o1.ms_device)
+        dev.append(o1.ms_device)
+        wall.append(o1.ms_total)
     per, op = mcr.run_batch(ps, mode=mcr.BATCH_PER_QUERY)
-    ms_b = float(np.median(times))
-    res = {"workload": BATCH["name"], "portfolios": len(ps), "total_paths": total,
-           "n_min": min(p["n"] if "n" in p else len(p["mu"]) for p in ps),
-           "n_max": max(len(p["mu"]) for p in ps),
+    mism = [i for i in range(len(ps)) if not same_result(r1[i], per[i])]
+    if mism:
+        raise SystemExit(f"batch leg: {len(mism)} batched results differ from the query alone "
+                         f"(first: {mism[:5]})")
+    ms_b, wall_b = float(np.median(dev)), float(np.median(wall))
+    res = {"workload": wl.CONFIG4["name"], "portfolios": len(ps), "total_paths": total,
+           "n_min": min(len(p["mu"]) for p in ps), "n_max": max(len(p["mu"]) for p in ps),
            "autotune": {"candidates": tried, "chunk_paths": best.chunk_used,
                         "order": "lpt" if best.order_used == 0 else "natural"},
-           "persistent_graph_ms": ms_b, "persistent_paths_per_s": total / (ms_b * 1e-3),
+           "persistent_graph_device_ms": ms_b, "persistent_graph_wall_ms": wall_b,
+           "persistent_paths_per_s": total / (ms_b * 1e-3),
            "graph_replayed": bool(o1.graph_replayed), "work_items": o1.work_items,
-           "per_portfolio_ms": op.ms_total, "per_portfolio_device_ms": op.ms_device,
-           "per_portfolio_paths_per_s": total / (op.ms_total * 1e-3),
-           "speedup_vs_per_portfolio": op.ms_total / ms_b}
-    print(f"# batch: persistent+graph {ms_b:.1f} ms, per-portfolio {op.ms_total:.1f} ms "
-          f"({total:.3g} paths)", file=sys.stderr, flush=True)
+           "per_portfolio_device_ms": op.ms_device, "per_portfolio_wall_ms": op.ms_total,
+           "per_portfolio_paths_per_s": total / (op.ms_device * 1e-3),
+           "speedup_device": op.ms_device / ms_b, "speedup_wall": op.ms_total / wall_b,
+           "results_equal_alone": True}
+    print(f"# batch: persistent+graph {ms_b:.1f} ms device / {wall_b:.1f} ms wall, per-portfolio "
+          f"{op.ms_device:.1f} ms device / {op.ms_total:.1f} ms wall ({total:.3g} paths); all "
+          f"{len(ps)} equal alone", file=sys.stderr, flush=True)
     return res
 
 
-def accuracy_check(mcr, np):
-    """bf16x3 MFMA vs FP64 on the identical stream (the oracle as checker),
-    on path samples of the n = 50 and n = 600 workloads: relative error of the
-    exceedance probability and of VaR99."""
-    import __graft_entry__ as ge
-
-    orc = ge.load_oracle()
-    threads = min(os.cpu_count() or 1, 16)
-    out = {}
-    for name, G, M in (("n50", dict(n=WORKLOAD["n"], K=WORKLOAD["factors"], T=WORKLOAD["days"],
-                                    ds=WORKLOAD["data_seed"], seed=WORKLOAD["path_seed"],
-                                    k=WORKLOAD["k"], x=WORKLOAD["x"]), 2_000_000),
-                       ("n600", dict(n=LARGE["n"], K=LARGE["factors"], T=LARGE["days"],
-                                     ds=LARGE["data_seed"], seed=LARGE["path_seed"], k=LARGE["k"],
-                                     x=LARGE["x"]), 100_000)):
-        n = G["n"]
-        mu, sigma = mcr.synth_factor_model(n, G["K"], G["T"], G["ds"])
-        r = mcr.run(mu, sigma, G["x"], G["k"], M, G["seed"], precision=mcr.PREC_BF16X3,
-                    alphas=(0.99,))
-        L, *_ = mcr.factor(sigma)
-        ref = orc.simulate_f64(L, mu, np.full(n, 1.0 / n), G["x"], G["seed"], 0, M, G["k"],
-                               want_loss=True, nthreads=threads)
-        rank = min(max(int(np.ceil(0.99 * M)), 1), M)
-        v64 = float(np.sort(ref["loss"])[rank - 1])
-        out[name] = {"paths": M, "kernel": r.kernel, "count_bf16x3": r.count,
-                     "count_fp64": ref["count"],
-                     "rel_err_p_hat": abs(r.count - ref["count"]) / max(ref["count"], 1),
-                     "rel_err_var99": abs(r.var[0] - v64) / abs(v64)}
-    return out
-
-
-def serve_queries(mcr, np, count, seed):
-    """Config 5 stream: arrival gaps ~ Exp(1/qps); sizes as config 4; a query
-    reuses an earlier query's covariance (and mean) with probability 0.3."""
-    from concurrent.futures import ThreadPoolExecutor
-
-    S = SERVE
-    rng = np.random.default_rng(seed)
-    shape, src = [], []
-    for i in range(count):
-        if i and rng.uniform() < S["reuse"]:
-            src.append(src[int(rng.integers(0, i))])
-        else:
-            src.append(i)
-        n = int(round(np.exp(rng.uniform(np.log(S["n_range"][0]), np.log(S["n_range"][1])))))
-        shape.append(n)
-    distinct = sorted(set(src))
-
-    def make(i):
-        n = shape[i]
-        return mcr.synth_factor_model(n, int(min(20, max(1, n // 30))), 2 * n + 200,
-                                      seed * 1000 + i)
-
-    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
-        data = dict(zip(distinct, ex.map(make, distinct)))
-    gaps = rng.exponential(1.0 / S["qps"], count)
-    out = []
-    for i in range(count):
-        mu, sigma = data[src[i]]
-        n = mu.shape[0]
-        m = int(round(np.exp(rng.uniform(np.log(S["paths_range"][0]), np.log(S["paths_range"][1])))))
-        out.append({"mu": mu, "sigma": sigma, "x": float(rng.uniform(*S["x_range"])),
-                    "k": max(1, (n + 9) // 10), "paths": m, "seed": int(rng.integers(0, 2**62)),
-                    "weights": rng.dirichlet(np.ones(n)), "alphas": (S["alpha"],),
-                    "precision": mcr.PREC_BF16X3})
-    return out, gaps, len(distinct)
-
-
-def serve_leg(mcr, np, seconds):
-    """Config 5: open-loop Poisson arrivals into the serving loop; latency is
-    submit -> result ready as the server records it."""
-    import threading
-
-    S = SERVE
-    count = max(1, int(S["qps"] * seconds))
-    qs, gaps, n_distinct = serve_queries(mcr, np, count, S["seed"])
-    warm, _, _ = serve_queries(mcr, np, 8, S["seed"] + 1)
-    infos = [None] * count
-    with mcr.Server() as srv:
-        # warm-up on other queries (kernel load, cost-model calibration)
-        for t in [srv.submit(q) for q in warm]:
-            srv.wait(t)
-        tickets = [None] * count
-        ready = threading.Semaphore(0)
-
-        def collect():
-            for i in range(count):
-                ready.acquire()
-                infos[i] = srv.wait(tickets[i])[1]
-
-        th = threading.Thread(target=collect)
-        th.start()
-        st0 = srv.stats()
-        t0 = time.perf_counter()
-        due = t0
-        for i in range(count):
-            due += gaps[i]
-            d = due - time.perf_counter()
-            if d > 0:
-                time.sleep(d)
-            tickets[i] = srv.submit(qs[i])
-            ready.release()
-        submit_span = time.perf_counter() - t0
-        th.join()
-        wall = time.perf_counter() - t0
-        st = srv.stats()
-    lat = np.array([x.latency_ms for x in infos])
-    que = np.array([x.queue_ms for x in infos])
+def serve_leg(mcr, np, wl, count, rates):
+    """Config 5: open-loop Poisson arrivals into the serving loop at each
+    offered rate (the same queries, arrival gaps scaled); latency is submit ->
+    result ready as the server records it."""
+    S = wl.CONFIG5
+    qs, gaps, n_distinct = wl.serve_queries(mcr, count)
+    warm, _, _ = wl.serve_queries(mcr, 8, S["seed"] + 1)
     paths = sum(q["paths"] for q in qs)
-    hits = st.cache_hits - st0.cache_hits
-    res = {"workload": S["name"], "queries": count, "offered_qps": S["qps"],
-           "achieved_qps": count / submit_span, "distinct_covariances": n_distinct,
-           "total_paths": paths, "sustained_paths_per_s": paths / wall,
-           "latency_ms": {"p50": float(np.percentile(lat, 50)), "p90": float(np.percentile(lat, 90)),
-                          "p99": float(np.percentile(lat, 99)), "max": float(lat.max()),
-                          "mean": float(lat.mean())},
-           "queue_ms": {"p50": float(np.percentile(que, 50)), "p99": float(np.percentile(que, 99))},
-           "rounds": st.rounds - st0.rounds,
-           "device_busy_frac": (st.device_ms - st0.device_ms) / (wall * 1e3),
-           "cache_hit_rate": hits / count, "peak_device_bytes": st.peak_bytes,
-           "budget_bytes": st.budget_bytes, "failed": st.failed}
-    print(f"# serve: {count} queries at {res['achieved_qps']:.0f} QPS, p50 {res['latency_ms']['p50']:.1f} ms,"
-          f" p99 {res['latency_ms']['p99']:.1f} ms, {res['sustained_paths_per_s']:.3g} paths/s, "
-          f"busy {res['device_busy_frac']:.2f}, cache hits {hits}", file=sys.stderr, flush=True)
+    points = []
+    for rate in rates:
+        with mcr.Server() as srv:
+            for t in [srv.submit(q) for q in warm]:  # kernel load, cost-model calibration
+                srv.wait(t)
+            st0 = srv.stats()
+            _, infos, span, wall = wl.replay_stream(srv, qs, gaps * (S["qps"] / rate))
+            st = srv.stats()
+        lat = np.array([x.latency_ms for x in infos])
+        que = np.array([x.queue_ms for x in infos])
+        pt = {"offered_qps": rate, "achieved_qps": count / span, "completed_qps": count / wall,
+              "queries": count, "latency_ms": {"p50": float(np.percentile(lat, 50)),
+                                               "p90": float(np.percentile(lat, 90)),
+                                               "p99": float(np.percentile(lat, 99)),
+                                               "max": float(lat.max()), "mean": float(lat.mean())},
+              "queue_ms": {"p50": float(np.percentile(que, 50)), "p99": float(np.percentile(que, 99))},
+              "sustained_paths_per_s": paths / wall,
+              "device_busy_frac": (st.device_ms - st0.device_ms) / (wall * 1e3),
+              "rounds": st.rounds - st0.rounds,
+              "cache_hit_rate": (st.cache_hits - st0.cache_hits) / count,
+              "peak_device_bytes": st.peak_bytes, "failed": st.failed - st0.failed}
+        points.append(pt)
+        print(f"# serve {rate:.0f} QPS offered: {pt['achieved_qps']:.0f} submitted/s, "
+              f"{pt['completed_qps']:.0f} done/s, p50 {pt['latency_ms']['p50']:.2f} ms, p99 "
+              f"{pt['latency_ms']['p99']:.1f} ms, {pt['sustained_paths_per_s']:.3g} paths/s, busy "
+              f"{pt['device_busy_frac']:.2f}", file=sys.stderr, flush=True)
+    head = points[0]
+    # saturation: the highest offered rate the loop still completes at >= 95 %
+    kept = [p for p in points if p["completed_qps"] >= 0.95 * p["offered_qps"]]
+    res = {"workload": S["name"], "queries": count, "distinct_covariances": n_distinct,
+           "total_paths_per_rate": paths, "offered_qps": head["offered_qps"],
+           "latency_ms": head["latency_ms"], "queue_ms": head["queue_ms"],
+           "sustained_paths_per_s": head["sustained_paths_per_s"],
+           "device_busy_frac": head["device_busy_frac"], "cache_hit_rate": head["cache_hit_rate"],
+           "peak_device_bytes": head["peak_device_bytes"], "failed": head["failed"],
+           "p99_bound_ms": wl.P99_BOUND_MS,
+           "p99_within_bound": head["latency_ms"]["p99"] <= wl.P99_BOUND_MS,
+           "max_rate_kept_qps": max(p["offered_qps"] for p in kept) if kept else None,
+           "peak_sustained_paths_per_s": max(p["sustained_paths_per_s"] for p in points),
+           "sweep": points}
     return res
 
 
-def large_leg(mcr, np):
+def large_leg(mcr, np, wl):
     """n = 600 (config 3): plain fp32 tile kernel vs bf16x3 MFMA, same stream."""
-    G = LARGE
-    mu, sigma = mcr.synth_factor_model(G["n"], G["factors"], G["days"], G["data_seed"])
+    G = wl.CONFIG3
+    mu, sigma = wl.model_of(mcr, G)
     res = {"workload": G["name"], "n_assets": G["n"], "paths": G["paths"],
            "cond_sigma": float(np.linalg.cond(sigma))}
     for name, prec in (("fp32", mcr.PREC_FP32), ("bf16x3", mcr.PREC_BF16X3)):
@@ -284,6 +264,30 @@ def large_leg(mcr, np):
     return res
 
 
+def accuracy_check(mcr, np, orc, wl):
+    """bf16x3 MFMA vs FP64 on the identical stream (the oracle as checker),
+    on path samples of the n = 50 and n = 600 workloads: relative error of the
+    exceedance probability and of VaR99 / VaR99.9."""
+    threads, _ = cpu_share()
+    out = {}
+    for name, C, M in (("n50", wl.CONFIG2, 2_000_000), ("n600", wl.CONFIG3, 200_000)):
+        n = C["n"]
+        mu, sigma = wl.model_of(mcr, C)
+        r = mcr.run(mu, sigma, C["x"], C["k"], M, C["path_seed"], precision=mcr.PREC_BF16X3,
+                    alphas=(0.99, 0.999))
+        L, *_ = mcr.factor(sigma)
+        ref = orc.simulate_f64(L, mu, np.full(n, 1.0 / n), C["x"], C["path_seed"], 0, M, C["k"],
+                               want_loss=True, nthreads=threads)
+        srt = np.sort(ref["loss"])
+        v64 = [float(srt[min(max(int(np.ceil(a * M)), 1), M) - 1]) for a in (0.99, 0.999)]
+        out[name] = {"paths": M, "kernel": r.kernel, "count_bf16x3": r.count,
+                     "count_fp64": ref["count"],
+                     "rel_err_p_hat": abs(r.count - ref["count"]) / max(ref["count"], 1),
+                     "rel_err_var99": abs(r.var[0] - v64[0]) / abs(v64[0]),
+                     "rel_err_var999": abs(r.var[1] - v64[1]) / abs(v64[1])}
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -296,6 +300,7 @@ def main():
     import __graft_entry__ as ge
 
     mcr = ge.load_package()
+    wl = mcr.workloads()
     if args.copy_mb:
         import torch
 
@@ -305,14 +310,14 @@ def main():
         torch.cuda.synchronize()
         print(f"# calibration kernel (neg): {args.copy_mb} MiB read + {args.copy_mb} MiB written",
               file=sys.stderr, flush=True)
-    W = WORKLOAD
+    W = wl.CONFIG2
     n = W["n"]
-    mu, sigma = mcr.synth_factor_model(n, W["factors"], W["days"], W["data_seed"])
+    mu, sigma = wl.model_of(mcr, W)
     prec = {"auto": mcr.PREC_AUTO, "fp32": mcr.PREC_FP32, "bf16x3": mcr.PREC_BF16X3}[args.precision]
 
-    def step(precision):
+    def step(precision, near=False):
         return mcr.run(mu, sigma, W["x"], W["k"], W["paths"], W["path_seed"], precision=precision,
-                       alphas=(W["alpha"],))
+                       alphas=(W["alpha"],), near=near)
 
     # the first warm-up query pays the library's lazy set-up (HIP context,
     # code objects, device arena, pinned staging): the cold figure a driver
@@ -346,13 +351,26 @@ def main():
     print(f"# step {ms_step:.3f} ms device, {wall_ms:.3f} ms wall, kernel {k_ms:.3f} ms ({r.kernel}), count={r.count} "
           f"p_hat={r.p_hat:.6f} VaR99={r.var[0]:.6f}", file=sys.stderr, flush=True)
 
-    # the plain fp32 fused kernel beside it, same workload
+    # the plain fp32 fused kernel beside it, same workload (its count, with the
+    # near-threshold count, is the one the CPU leg compares at full size)
     extra = {}
-    if r.kernel != "valu_reg":
-        f = [step(mcr.PREC_FP32) for _ in range(3)]
-        extra["fp32_kernel_ms"] = float(np.mean([x.ms_kernel for x in f]))
-        extra["fp32_paths_per_s"] = W["paths"] / (extra["fp32_kernel_ms"] * 1e-3)
-        extra["fp32_count"] = f[-1].count
+    f = [step(mcr.PREC_FP32) for _ in range(3)]
+    fnear = step(mcr.PREC_FP32, near=True)
+    extra["fp32_kernel"] = f[-1].kernel
+    extra["fp32_kernel_ms"] = float(np.mean([x.ms_kernel for x in f]))
+    extra["fp32_paths_per_s"] = W["paths"] / (float(np.mean([x.ms_device for x in f])) * 1e-3)
+    extra["fp32_count"] = fnear.count
+    extra["fp32_near_threshold"] = fnear.near_threshold
+    extra["speedup_vs_fp32_kernel"] = extra["fp32_kernel_ms"] / k_ms
+    roof = {"bound": "mfma" if is_mfma else "valu", "achieved": achieved, "peak": peak,
+            "unit": "TFLOP/s", "frac": achieved / peak,
+            "traffic": TRAFFIC_BYTES.get(r.kernel), "kernel": r.kernel, "kernel_ms": k_ms,
+            "flop_per_path": n * (n + 1),
+            "note": ("achieved counts the mat-vec FLOPs only; the kernel is bound by the VALU "
+                     "issue of normal generation (Philox4x32-10, Box-Muller), which valu "
+                     "gives from rocprof counters")}
+    if VALU_PROFILE.get(r.kernel):
+        roof["valu"] = VALU_PROFILE[r.kernel]
     out = {
         "metric": "paths/sec at n=50 & n=600 vs plain FP32 kernel; p50/p99 latency at fixed QPS",
         "value": value, "unit": "paths/s", "n_gpus": 1, "steps": args.steps,
@@ -365,13 +383,7 @@ def main():
         "config": {"workload": W["name"], "n_assets": n, "paths": W["paths"], "k": W["k"],
                    "x": W["x"], "var_alpha": W["alpha"], "kernel": r.kernel,
                    "parallelism": "single GPU"},
-        "roofline": {"bound": "mfma" if is_mfma else "valu", "achieved": achieved, "peak": peak,
-                     "unit": "TFLOP/s", "frac": achieved / peak,
-                     "traffic": TRAFFIC_BYTES.get(r.kernel), "kernel": r.kernel, "kernel_ms": k_ms,
-                     "flop_per_path": n * (n + 1),
-                     "note": ("achieved counts the mat-vec FLOPs only; the kernel is bound by the "
-                              "VALU issue of normal generation (Philox4x32-10, Box-Muller: "
-                              "~83 % of its instructions), see profiles/k_mfma_n50_100M.md")},
+        "roofline": roof,
         "result": {"count": r.count, "p_hat": r.p_hat, "var99": r.var[0], "es99": r.es[0]},
         "wall_ms_per_step": wall_ms,
         "cold_first_query_ms": cold_ms,
@@ -380,17 +392,21 @@ def main():
         "normals_per_s": W["paths"] * n / (k_ms * 1e-3),
     }
     out.update(extra)
+    orc = None if (args.no_cpu and args.no_plain) else ge.load_oracle()
+    if not args.no_plain:
+        out["plain"] = plain_leg(mcr, orc, wl)
     if not args.no_large:
-        out["large"] = large_leg(mcr, np)
+        out["large"] = large_leg(mcr, np, wl)
     if not args.no_batch:
-        out["batch"] = batch_leg(mcr, np)
+        out["batch"] = batch_leg(mcr, np, wl)
     if not args.no_serve:
-        out["serving"] = serve_leg(mcr, np, args.serve_seconds)
+        rates = [float(x) for x in args.serve_rates.split(",") if x]
+        out["serving"] = serve_leg(mcr, np, wl, args.serve_queries, rates)
     if not args.no_cpu:
-        L, *_ = mcr.factor(sigma)
-        out["cpu_baseline"] = cpu_baseline(L, mu, W["x"], W["k"], W["path_seed"], n,
-                                           args.cpu_seconds)
-        out["accuracy_vs_fp64"] = accuracy_check(mcr, np)
+        cb = cpu_baseline(mcr, orc, W, mu, sigma, fnear, args.cpu_serial_paths)
+        cb["gpu_vs_openmp"] = value / cb["value"]
+        out["cpu_baseline"] = cb
+        out["accuracy_vs_fp64"] = accuracy_check(mcr, np, orc, wl)
         print(f"# accuracy vs FP64: {out['accuracy_vs_fp64']}", file=sys.stderr, flush=True)
     print(json.dumps(out), flush=True)
 

@@ -204,8 +204,19 @@ def _make_query(mu, sigma, x, k, paths, seed, weights=None, precision=PREC_FP32,
     """A filled Query plus the arrays it points into (keep them alive)."""
     mu = np.ascontiguousarray(mu, dtype=np.float64)
     sigma = np.ascontiguousarray(sigma, dtype=np.float64)
+    if mu.ndim != 1:
+        raise ValueError(f"mu must be one-dimensional (shape {mu.shape})")
     n = int(mu.shape[0])
+    # the library copies n rows of ld_sigma doubles and n weights: the arrays
+    # must hold them
+    if sigma.ndim == 2:
+        if sigma.shape[0] < n or sigma.shape[1] < n:
+            raise ValueError(f"sigma of shape {sigma.shape} is smaller than {n} x {n}")
+    elif sigma.ndim != 1 or sigma.size < n * n:
+        raise ValueError(f"sigma must be {n} x {n} (or hold {n * n} values flat)")
     w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+    if w is not None and w.shape != (n,):
+        raise ValueError(f"weights must have shape ({n},), not {w.shape}")
     S = np.zeros(paths, dtype=np.int32) if want_S else None
     loss = np.zeros(paths, dtype=np.float32) if want_loss else None
     hist = np.zeros(n + 1, dtype=np.int64) if want_hist else None
@@ -421,6 +432,8 @@ def factor(sigma):
     """GPU FP64 Cholesky with the jitter policy: (L, attempts, jitter, clamped)."""
     sigma = np.ascontiguousarray(sigma, dtype=np.float64)
     n = sigma.shape[0]
+    if sigma.shape != (n, n):
+        raise ValueError(f"sigma must be square, not {sigma.shape}")
     L = np.zeros((n, n), dtype=np.float64)
     info = FactorInfo()
     _check(lib().mcr_factor(n, _ptr(sigma, _f64p), n, _ptr(L, _f64p), ctypes.byref(info)))
@@ -432,7 +445,11 @@ def prepare(mu, sigma, x: float, weights=None) -> dict:
     mu = np.ascontiguousarray(mu, dtype=np.float64)
     sigma = np.ascontiguousarray(sigma, dtype=np.float64)
     n = mu.shape[0]
+    if mu.ndim != 1 or sigma.shape != (n, n):
+        raise ValueError(f"mu must be ({n},) and sigma {n} x {n} (got {mu.shape}, {sigma.shape})")
     w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+    if w is not None and w.shape != (n,):
+        raise ValueError(f"weights must have shape ({n},), not {w.shape}")
     q = Query()
     q.n, q.mu, q.sigma, q.ld_sigma = n, _ptr(mu, _f64p), _ptr(sigma, _f64p), n
     q.weights, q.x, q.k, q.paths = _ptr(w, _f64p), float(x), 1, 1
@@ -669,4 +686,11 @@ def preprocessing():
     import importlib
 
     return importlib.import_module(__name__ + ".preprocess")
+
+
+def workloads():
+    """The seeded benchmark workloads (workloads.py)."""
+    import importlib
+
+    return importlib.import_module(__name__ + ".workloads")
 

@@ -190,8 +190,12 @@ size_t layout(const Plan& p, char* base, DevBuffers* b) {
 int choose_kernel(const mcr_query* q, Plan* p) {
   const int n = q->n;
   int k = q->kernel;
+  // The near-threshold count (paths with some r_i within 1 ulp of the fp32
+  // threshold) is kept by the fp32 kernels only: the bf16x3 products carry an
+  // error of several ulp, so a 1-ulp window there would name no definite set.
+  const bool near = (q->flags & MCR_F_NEAR) != 0;
   if (k == MCR_KERNEL_AUTO) {
-    if (q->precision == MCR_PREC_FP32) {
+    if (q->precision == MCR_PREC_FP32 || (q->precision == MCR_PREC_AUTO && near)) {
       k = n <= kRegMaxN ? MCR_KERNEL_VALU_REG : MCR_KERNEL_VALU_TILE;
     } else if (q->precision == MCR_PREC_BF16X3 || q->precision == MCR_PREC_AUTO) {
       k = n <= 32 * kMfmaMaxT - 1 ? MCR_KERNEL_MFMA : MCR_KERNEL_MFMA_LDS;
@@ -211,6 +215,11 @@ int choose_kernel(const mcr_query* q, Plan* p) {
   if (k < MCR_KERNEL_VALU_REG || k > MCR_KERNEL_MFMA_LDS) {
     set_error("unknown kernel %d", k);
     return MCR_E_ARG;
+  }
+  if (near && (k == MCR_KERNEL_MFMA || k == MCR_KERNEL_MFMA_LDS)) {
+    set_error("MCR_F_NEAR needs an fp32 kernel (precision MCR_PREC_FP32 or MCR_PREC_AUTO); "
+              "the bf16x3 MFMA kernels keep no near-threshold count");
+    return MCR_E_UNSUPPORTED;
   }
   p->kernel = k;
   if (k == MCR_KERNEL_MFMA || k == MCR_KERNEL_MFMA_LDS) {
@@ -997,7 +1006,9 @@ int run_batch_locked(const mcr_query* qs, int nq, mcr_result* rs, mcr_batch_opts
     for (int i = 0; i < nq; ++i) {
       int r = run_locked(&qs[i], &rs[i]);
       if (r != MCR_OK) return r;
-      dev_ms += rs[i].ms_kernel;
+      // factor, layouts and path kernels of the query (HIP events), the
+      // same device work the persistent mode times
+      dev_ms += rs[i].ms_device;
     }
     o->ms_device = dev_ms;
     o->ms_total =

@@ -1,0 +1,169 @@
+"""The five benchmark workloads as seeded generators (host only).
+
+Each returns plain dicts in the form ``run_batch`` / ``Server.submit`` take
+(mu, sigma, x, k, paths, seed, weights, alphas, precision), so the benchmark
+and the tests build exactly the same queries.  Covariances come from the
+seeded factor model of ``synth_factor_model`` (r_t = 2e-4 + B f_t + e_t,
+B ~ U(0.2, 0.8), 1 %/day factor and idiosyncratic vol; sample mean and ddof-1
+covariance, as src/preprocessing/preprocessing_stats.py:5-28 computes them).
+"""
+from __future__ import annotations
+
+import os
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+
+# Plain method, small: n = 20 assets, 3 factors over 1000 days, 1M paths, k = 5.
+CONFIG1 = {"name": "plain_n20_1M", "n": 20, "factors": 3, "days": 1000, "paths": 1_000_000,
+           "k": 5, "x": 0.02, "data_seed": 1001, "path_seed": 20251201}
+# Fused kernel at scale: n = 50, 5 factors over 2000 days, 100M paths, k = 10, VaR99.
+CONFIG2 = {"name": "fused_n50_100M", "n": 50, "factors": 5, "days": 2000, "paths": 100_000_000,
+           "k": 10, "x": 0.02, "alpha": 0.99, "data_seed": 1002, "path_seed": 20251202}
+# Large n: n = 600, 20 factors over 2500 days (cond ~1e4), 10M paths, VaR99 / VaR99.9.
+CONFIG3 = {"name": "large_n600_10M", "n": 600, "factors": 20, "days": 2500,
+           "paths": 10_000_000, "k": 10, "x": 0.02, "alphas": (0.99, 0.999),
+           "data_seed": 1003, "path_seed": 20251203}
+# 256 uneven portfolios in one batch.
+CONFIG4 = {"name": "batch_256_uneven", "count": 256, "n_range": (5, 600),
+           "paths_range": (1e5, 1e7), "x_range": (0.01, 0.04), "alpha": 0.99, "seed": 1004}
+# Poisson query stream at a fixed rate, sizes as config 4, 30 % covariance reuse.
+CONFIG5 = {"name": "serve_poisson_200qps", "qps": 200.0, "reuse": 0.3, "n_range": (5, 600),
+           "paths_range": (1e5, 1e7), "x_range": (0.01, 0.04), "alpha": 0.99, "seed": 1005}
+
+
+# Stated latency bound of the serving loop at CONFIG5's rate: p99 of submit ->
+# result ready.  The largest queries (n ~ 600, 1e7 paths) need ~15 ms of device
+# time alone, so the bound leaves them room to share the device with the rest.
+P99_BOUND_MS = 50.0
+
+
+def model_of(mcr, cfg):
+    """(mu, sigma) of a single-query config (CONFIG1-3)."""
+    return mcr.synth_factor_model(cfg["n"], cfg["factors"], cfg["days"], cfg["data_seed"])
+
+
+def _log_uniform(rng, lo, hi):
+    return float(np.exp(rng.uniform(np.log(lo), np.log(hi))))
+
+
+def _factors_for(n):
+    return int(min(20, max(1, n // 30)))
+
+
+def _pool():
+    # synth_factor_model runs an OpenMP team per call: a few callers at once
+    # keep the host's threads to a few times OMP_NUM_THREADS
+    return ThreadPoolExecutor(max_workers=max(1, min(4, os.cpu_count() or 1)))
+
+
+def batch_portfolios(mcr, count=None, seed=None):
+    """Config 4: n log-uniform in [5, 600], paths log-uniform in [1e5, 1e7],
+    Dirichlet(1) weights, thresholds U(1 %, 4 %), k = ceil(n / 10); Sigma from
+    the factor model with K = clamp(n / 30, 1, 20) factors over T = 2n + 200
+    days."""
+    B = CONFIG4
+    seed = B["seed"] if seed is None else seed
+    rng = np.random.default_rng(seed)
+    spec = []
+    for i in range(count or B["count"]):
+        n = int(round(_log_uniform(rng, *B["n_range"])))
+        m = int(round(_log_uniform(rng, *B["paths_range"])))
+        spec.append((i, n, m, float(rng.uniform(*B["x_range"])), int(rng.integers(0, 2**62)),
+                     rng.dirichlet(np.ones(n))))
+
+    def make(s):
+        i, n = s[0], s[1]
+        return mcr.synth_factor_model(n, _factors_for(n), 2 * n + 200, seed * 1000 + i)
+
+    with _pool() as ex:
+        data = list(ex.map(make, spec))
+    out = []
+    for (i, n, m, x, ps, w), (mu, sigma) in zip(spec, data):
+        out.append({"mu": mu, "sigma": sigma, "x": x, "k": max(1, (n + 9) // 10), "paths": m,
+                    "seed": ps, "weights": w, "alphas": (B["alpha"],),
+                    "precision": mcr.PREC_BF16X3})
+    return out
+
+
+def serve_queries(mcr, count, seed=None, qps=None):
+    """Config 5 stream: arrival gaps ~ Exp(1 / qps); sizes as config 4; a query
+    reuses an earlier query's covariance (and mean) with probability 0.3.
+    Returns (queries, gaps in seconds, number of distinct covariances)."""
+    S = CONFIG5
+    seed = S["seed"] if seed is None else seed
+    qps = S["qps"] if qps is None else qps
+    rng = np.random.default_rng(seed)
+    shape, src = [], []
+    for i in range(count):
+        if i and rng.uniform() < S["reuse"]:
+            src.append(src[int(rng.integers(0, i))])
+        else:
+            src.append(i)
+        shape.append(int(round(_log_uniform(rng, *S["n_range"]))))
+    distinct = sorted(set(src))
+
+    def make(i):
+        n = shape[i]
+        return mcr.synth_factor_model(n, _factors_for(n), 2 * n + 200, seed * 1000 + i)
+
+    with _pool() as ex:
+        data = dict(zip(distinct, ex.map(make, distinct)))
+    gaps = rng.exponential(1.0 / qps, count)
+    out = []
+    for i in range(count):
+        mu, sigma = data[src[i]]
+        n = mu.shape[0]
+        m = int(round(_log_uniform(rng, *S["paths_range"])))
+        out.append({"mu": mu, "sigma": sigma, "x": float(rng.uniform(*S["x_range"])),
+                    "k": max(1, (n + 9) // 10), "paths": m, "seed": int(rng.integers(0, 2**62)),
+                    "weights": rng.dirichlet(np.ones(n)), "alphas": (S["alpha"],),
+                    "precision": mcr.PREC_BF16X3})
+    return out, gaps, len(distinct)
+
+
+def replay_stream(srv, qs, gaps, on_result=None):
+    """Open-loop replay of a query stream into a Server: query i is submitted
+    gaps[0] + ... + gaps[i] seconds after the start, whatever the server's
+    progress; a collector thread waits for the results in order.  Returns
+    (results, ticket infos, submit span in s, wall span in s); on_result(i,
+    result, info) is called as each result is collected."""
+    import threading
+    import time
+
+    count = len(qs)
+    tickets, results, infos = [None] * count, [None] * count, [None] * count
+    ready = threading.Semaphore(0)
+    errors = []
+
+    def collect():
+        for i in range(count):
+            ready.acquire()
+            try:
+                results[i], infos[i] = srv.wait(tickets[i])
+                if on_result is not None:
+                    on_result(i, results[i], infos[i])
+            except Exception as e:  # keep collecting; report after the join
+                errors.append((i, e))
+
+    th = threading.Thread(target=collect)
+    th.start()
+    t0 = time.perf_counter()
+    due = t0
+    try:
+        for i in range(count):
+            due += gaps[i]
+            d = due - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            tickets[i] = srv.submit(qs[i])
+            ready.release()
+    finally:
+        submit_span = time.perf_counter() - t0
+        for _ in range(count - sum(t is not None for t in tickets)):
+            ready.release()  # unblock the collector after a failed submit
+        th.join()
+    wall = time.perf_counter() - t0
+    if errors:
+        raise errors[0][1]
+    return results, infos, submit_span, wall

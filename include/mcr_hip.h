@@ -54,7 +54,9 @@ extern "C" {
 
 /* flags */
 #define MCR_F_VAR 0x1   /* portfolio-loss quantiles (VaR) and tail sums at alpha[] */
-#define MCR_F_NEAR 0x2  /* count paths with some r_i within 1 ulp of the threshold */
+#define MCR_F_NEAR 0x2  /* count paths with some r_i within 1 ulp of the threshold; fp32
+                           kernels only (MCR_PREC_AUTO then picks fp32, MCR_PREC_BF16X3
+                           or an MFMA kernel fails with MCR_E_UNSUPPORTED) */
 #define MCR_F_SHIST 0x4 /* histogram of S (n+1 bins) into s_hist */
 /* test hook: zero-width VaR windows, forcing the quantile refinement passes */
 #define MCR_F_TEST_NARROW 0x100
@@ -153,7 +155,8 @@ typedef struct mcr_batch_opts {
   int order;         /* MCR_ORDER_*; used when chunk_paths != 0 */
   int grid;          /* persistent workgroups per launch; 0 = each class kernel's
                         resident maximum (occupancy x CUs) */
-  double ms_device;  /* out: device time of the batch (HIP events) */
+  double ms_device;  /* out: device time of the batch (HIP events): factor, layouts and
+                        path kernels; per-query mode sums each query's ms_device */
   double ms_total;   /* out: host wall time of the call */
   int chunk_used;    /* out: chunk_paths actually used */
   int order_used;    /* out */

@@ -47,33 +47,35 @@ typedef struct {
 } gsl_matrix;
 #endif
 
-/* Holds the final results of the Monte Carlo simulation */
+/* What a model hands back to the driver after one run_model call. */
 typedef struct {
-  double P_hat;     /* Estimated probability of extreme event */
-  int count;        /* Number of trials that exceeded threshold k */
-  int *S_values;    /* Crash count S per trial (malloc'd by the model) or NULL */
-  double std_error; /* Standard error of the estimate */
-  double ci_lower;  /* Lower bound of 95% confidence interval */
-  double ci_upper;  /* Upper bound of 95% confidence interval */
+  double P_hat;     /* count / M: share of paths with S >= k */
+  int count;        /* paths whose crash tally S reached k */
+  int *S_values;    /* optional per-path S array, malloc'd here and free()d by
+                       the driver; NULL when the model keeps none */
+  double std_error; /* sqrt(P_hat (1 - P_hat) / M) */
+  double ci_lower;  /* P_hat - 1.96 std_error, floored at 0 */
+  double ci_upper;  /* P_hat + 1.96 std_error, capped at 1 */
 } MonteCarloResult;
 
-/* Input parameters for the Monte Carlo simulation */
+/* One query as the driver fills it; the model reads it and never frees it. */
 typedef struct {
-  double x;                  /* Crash threshold (0.02 = 2%), compare R_i < -x */
-  int N;                     /* Number of assets */
-  int k;                     /* Minimum number of crashes for an extreme event */
-  gsl_vector *mu;            /* Mean returns of assets */
-  gsl_matrix *Sigma;         /* Covariance matrix of asset returns */
-  int M;                     /* Number of Monte Carlo trials to run */
-  unsigned long random_seed; /* Random seed for reproducibility */
+  double x;                  /* loss level: asset i crashes on a path iff r_i < -x */
+  int N;                     /* asset count, the length of mu and order of Sigma */
+  int k;                     /* a path is extreme once at least k assets crash */
+  gsl_vector *mu;            /* daily mean return per asset (may be strided) */
+  gsl_matrix *Sigma;         /* N x N return covariance, row stride tda */
+  int M;                     /* Monte Carlo path count */
+  unsigned long random_seed; /* seeds the path generator (the hip model keys
+                                Philox with it) */
 } MonteCarloParams;
 
-/* Common interface that all models implement: 0 on success, non-zero on failure */
+/* The entry every model exposes: 0 when the run completed, non-zero otherwise */
 typedef int (*ModelRunFunc)(MonteCarloParams *params, MonteCarloResult *result);
 
 typedef struct {
-  const char *name;       /* Model name ("serial", "openmp", "cuda", "hip") */
-  ModelRunFunc run_model; /* Main simulation function */
+  const char *name;       /* the key the driver's config lists ("hip" here) */
+  ModelRunFunc run_model; /* runs one query synchronously */
 } ModelFunctions;
 
 #endif /* MODEL_INTERFACE_H */

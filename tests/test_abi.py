@@ -66,6 +66,27 @@ def test_bad_arguments_rejected_without_gpu(mcr):
         mcr.synth_factor_model(0, 2, 100, 1)
 
 
+def test_misshaped_arrays_rejected_before_the_library(mcr):
+    """The C side copies n rows of ld_sigma doubles and n weights from the
+    caller's pointers; arrays too small for that must fail in Python."""
+    mu, sigma = np.zeros(6), np.eye(6)
+    ok, _ = mcr._make_query(mu, sigma, 0.02, 2, 64, 1)
+    assert ok.n == 6 and ok.ld_sigma == 6
+    strided, _ = mcr._make_query(mu, np.eye(6, 9), 0.02, 2, 64, 1)
+    assert strided.ld_sigma == 9
+    for bad in (dict(sigma=np.eye(5)), dict(sigma=np.eye(6)[:5]), dict(sigma=np.zeros(35)),
+                dict(weights=np.ones(5) / 5), dict(weights=np.ones((6, 1))),
+                dict(mu=np.zeros((6, 1)))):
+        args = dict(mu=mu, sigma=sigma, weights=None)
+        args.update(bad)
+        with pytest.raises(ValueError):
+            mcr._make_query(args["mu"], args["sigma"], 0.02, 2, 64, 1, weights=args["weights"])
+    with pytest.raises(ValueError):
+        mcr.factor(np.eye(6)[:5])
+    with pytest.raises(ValueError):
+        mcr.prepare(mu, np.eye(5), 0.02)
+
+
 def test_server_footprint_and_servable_checks(mcr):
     """Host-only admission arithmetic of the serving loop."""
     n = 600
@@ -89,3 +110,4 @@ def test_server_footprint_and_servable_checks(mcr):
     with pytest.raises(mcr.McrError) as e:
         mcr.server_footprint(dict(p, mu=mu))
     assert e.value.code == mcr.MCR_E_ARG
+

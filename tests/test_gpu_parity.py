@@ -124,7 +124,7 @@ def test_register_kernel_exact(mcr, orc, n):
     assert r.kernel == "valu_reg"
 
 
-@pytest.mark.parametrize("n", [1, 5, 8, 9, 50, 64, 65, 100, 129, 257, 600])
+@pytest.mark.parametrize("n", [1, 5, 8, 9, 50, 64, 65, 100, 129, 257, 600, 608])
 def test_tile_kernel_exact(mcr, orc, n):
     M = 20000 if n <= 129 else 4000
     r = _exact_case(mcr, orc, n, M, max(1, n // 5), mcr.KERNEL_VALU_TILE)

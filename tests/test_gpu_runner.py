@@ -40,6 +40,26 @@ def read_rows(path):
         return list(csv.DictReader(f))
 
 
+def test_runner_real_data_row_matches_oracle(mcr, orc, tmp_path):
+    """The CSV row's count against the C oracle (the serial model restated) on
+    the same real-data query and Philox stream, with the device's fp32 model:
+    identical integers, and the oracle's own FP64 statistics."""
+    M = 2_000_000
+    g, mu, S = setup_run(mcr, tmp_path, M, models="[hip]")
+    r = subprocess.run([mcr.RUNNER_PATH], cwd=tmp_path, input="oracle check\n",
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    row = read_rows(tmp_path / "results" / "simulation_results.csv")[0]
+    c = g["config"]
+    m = mcr.prepare(mu, S, c["x"])
+    model = orc.F32Model(m["L32"], m["mu32"], m["v32"], m["mp32"], m["thr32"])
+    ref = orc.simulate_f32(model, c["random_seed"], 0, M, c["k"], nthreads=16)
+    assert int(row["count"]) == ref["count"], (row["count"], ref["count"])
+    p, se, lo, hi = orc.stats(ref["count"], M)
+    assert row["P_hat"] == f"{p:.6f}" and row["std_error"] == f"{se:.6f}"
+    assert row["ci_lower"] == f"{lo:.6f}" and row["ci_upper"] == f"{hi:.6f}"
+
+
 def test_runner_real_data_row_matches_mcr_run(mcr, tmp_path):
     M = 4_000_000
     g, mu, S = setup_run(mcr, tmp_path, M)
