@@ -25,7 +25,9 @@ from dataclasses import dataclass, field
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "_lib", "libmcr_hip.so")
+# MCR_LIB_PATH selects another build of the same library (the host-sanitizer
+# variant of tests/sanitize.sh); by default the in-tree build
+LIB_PATH = os.environ.get("MCR_LIB_PATH") or os.path.join(_HERE, "_lib", "libmcr_hip.so")
 HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "mcr_hip.h")
 HEADER_PATHS = [HEADER_PATH, os.path.join(os.path.dirname(_HERE), "include", "mcr_io.h")]
 
@@ -166,6 +168,20 @@ def declared_functions() -> list[str]:
 def _check(code: int) -> None:
     if code != MCR_OK:
         raise McrError(code, lib().mcr_last_error().decode())
+
+
+def _square(sigma) -> np.ndarray:
+    """sigma as a contiguous n x n float64 array: a square 2-D array, or n*n
+    values flat (numpy's covariance of one variable is 0-d, i.e. 1 x 1)."""
+    s = np.ascontiguousarray(sigma, dtype=np.float64)
+    if s.ndim == 1:
+        n = math.isqrt(s.size)
+        if n * n != s.size:
+            raise ValueError(f"sigma with {s.size} values is not n x n")
+        return s.reshape(n, n)
+    if s.ndim != 2 or s.shape[0] != s.shape[1]:
+        raise ValueError(f"sigma must be square, not {s.shape}")
+    return s
 
 
 def _ptr(a, t):
@@ -430,10 +446,8 @@ class Server:
 
 def factor(sigma):
     """GPU FP64 Cholesky with the jitter policy: (L, attempts, jitter, clamped)."""
-    sigma = np.ascontiguousarray(sigma, dtype=np.float64)
+    sigma = _square(sigma)
     n = sigma.shape[0]
-    if sigma.shape != (n, n):
-        raise ValueError(f"sigma must be square, not {sigma.shape}")
     L = np.zeros((n, n), dtype=np.float64)
     info = FactorInfo()
     _check(lib().mcr_factor(n, _ptr(sigma, _f64p), n, _ptr(L, _f64p), ctypes.byref(info)))
@@ -443,7 +457,7 @@ def factor(sigma):
 def prepare(mu, sigma, x: float, weights=None) -> dict:
     """The fp32 model the path kernels read (factor computed on the GPU)."""
     mu = np.ascontiguousarray(mu, dtype=np.float64)
-    sigma = np.ascontiguousarray(sigma, dtype=np.float64)
+    sigma = _square(sigma)
     n = mu.shape[0]
     if mu.ndim != 1 or sigma.shape != (n, n):
         raise ValueError(f"mu must be ({n},) and sigma {n} x {n} (got {mu.shape}, {sigma.shape})")

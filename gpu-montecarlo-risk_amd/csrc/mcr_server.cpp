@@ -60,6 +60,7 @@ constexpr int kDefaultCache = 256;
 constexpr int kDefaultActive = 1024;
 constexpr double kDefaultRoundMs = 4.0;
 constexpr uint32_t kMaxItems = 1u << 16;  // work items per round
+constexpr uint64_t kItemGroups = 32;      // 64-path groups per work item, at most
 constexpr int kMaxRuns = 64;              // kernel classes per round (at most 41 exist)
 constexpr int kMaxBypass = 16;            // rounds a blocked head query may be overtaken
 constexpr double kInitMsPerUnit = 7e-5;   // initial device ms per cost unit per workgroup
@@ -564,12 +565,24 @@ int mcr_server::plan_round(uint64_t round, RoundRec* rr, std::vector<WorkItem>* 
     total_units += (double)g * c;
     grant.push_back({j, g});
   }
-  // item size: about six items per workgroup, at least one group
-  const double item_units = std::max(total_units / (6.0 * grid), 1.0);
+  // Item size per kernel class, as batches size them: at least four items
+  // per resident slot of the class's launch (a workgroup; a wave for the
+  // small-n classes, whose waves take items on their own) and at most
+  // kItemGroups groups.  Sizing from the round's total instead gave a class
+  // with little work a few huge items, one wave or workgroup grinding through
+  // each while the rest of the device idled inside that launch.
+  std::map<BatchKernel, uint64_t> class_groups;
+  for (auto& gr : grant) class_groups[class_kernel(gr.first->q.n)] += gr.second;
+  auto item_groups = [&](const Job* j) {
+    const int n = j->q.n;
+    const BatchKernel k = class_kernel(n);
+    const uint64_t slots =
+        (uint64_t)class_grid(k, class_block(n)) * (n <= kSmallMaxN ? 4 : 1);
+    return std::max<uint64_t>(1, std::min<uint64_t>(kItemGroups, class_groups[k] / (4 * slots)));
+  };
   size_t est = 0;
   for (auto& gr : grant) {
-    const double c = group_cost(gr.first->T, gr.first->S);
-    const uint64_t ig = std::max<uint64_t>(1, (uint64_t)(item_units / c));
+    const uint64_t ig = item_groups(gr.first);
     est += (size_t)((gr.second + ig - 1) / ig);
   }
   const double scale = est > kMaxItems ? (double)est / kMaxItems + 1.0 : 1.0;
@@ -577,7 +590,7 @@ int mcr_server::plan_round(uint64_t round, RoundRec* rr, std::vector<WorkItem>* 
   for (auto& gr : grant) {
     Job* j = gr.first;
     const double c = group_cost(j->T, j->S);
-    const uint64_t ig = std::max<uint64_t>(1, (uint64_t)(scale * item_units / c));
+    const uint64_t ig = std::max<uint64_t>(1, (uint64_t)std::ceil(scale * (double)item_groups(j)));
     const uint64_t g0 = j->next_group, g1 = g0 + gr.second;
     for (uint64_t g = g0; g < g1; g += ig) {
       WorkItem w;

@@ -13,7 +13,8 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+# ORACLE_LIB_PATH selects the sanitizer build (tests/sanitize.sh)
+LIB_PATH = os.environ.get("ORACLE_LIB_PATH") or os.path.join(_HERE, "_build", "liboracle.so")
 
 _lib = None
 
