@@ -137,6 +137,7 @@ def lib() -> ctypes.CDLL:
         L.mcr_init.argtypes = [ctypes.c_int]
         L.mcr_last_error.restype = ctypes.c_char_p
         L.mcr_run.argtypes = [ctypes.POINTER(Query), ctypes.POINTER(Result)]
+        L.mcr_run_noalloc.argtypes = [ctypes.POINTER(Query), ctypes.POINTER(Result)]
         L.mcr_factor.argtypes = [ctypes.c_int, _f64p, ctypes.c_int, _f64p,
                                  ctypes.POINTER(FactorInfo)]
         L.mcr_prepare.argtypes = [ctypes.POINTER(Query), ctypes.POINTER(ModelExport)]
@@ -288,6 +289,25 @@ def run(mu, sigma, x: float, k: int, paths: int, seed: int = 0, *, weights=None,
                           want_S, want_loss, want_hist, path_offset, extra_flags)
     r = Result()
     _check(lib().mcr_run(ctypes.byref(q), ctypes.byref(r)))
+    return _result(r, len(alphas), keep)
+
+
+def init(device: int = 0) -> None:
+    """Initialises the library on a device (mcr_init); idempotent."""
+    _check(lib().mcr_init(int(device)))
+
+
+def run_noalloc(mu, sigma, x: float, k: int, paths: int, seed: int = 0, *, weights=None,
+                precision: int = PREC_FP32, kernel: int = KERNEL_AUTO, alphas=(),
+                near: bool = False, want_S: bool = False, want_loss: bool = False,
+                want_hist: bool = False, path_offset: int = 0,
+                extra_flags: int = 0) -> RiskResult:
+    """One risk query through the allocation-free entry point (mcr_run_noalloc):
+    the library's static device scratch, no allocation; needs init() first."""
+    q, keep = _make_query(mu, sigma, x, k, paths, seed, weights, precision, kernel, alphas, near,
+                          want_S, want_loss, want_hist, path_offset, extra_flags)
+    r = Result()
+    _check(lib().mcr_run_noalloc(ctypes.byref(q), ctypes.byref(r)))
     return _result(r, len(alphas), keep)
 
 

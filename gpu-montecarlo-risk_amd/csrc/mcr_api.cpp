@@ -43,6 +43,25 @@ void set_error(const char* fmt, ...) {
 constexpr uint64_t kLaunchPaths = 1ull << 32;   // paths per launch (keeps lane tallies 32-bit)
 constexpr uint64_t kPerPathChunk = 1ull << 24;  // per-path outputs staged per launch
 
+// Device scratch of the allocation-free entry point (mcr_run_noalloc): every
+// device buffer of one query (covariance, factor, layouts, accumulators, VaR
+// gather buffers, per-path output chunks) is carved from this array instead of
+// the arena.  16 MiB holds the largest query (15.3 MiB: n = 608, four VaR levels,
+// per-path outputs staged 2^18 paths at a time); every device the library
+// runs on holds it until the process ends.  Host code reaches it only through
+// g_ctx.scratch, its device address from hipGetSymbolAddress.
+constexpr size_t kScratchBytes = 16u << 20;
+constexpr uint64_t kNoAllocChunk = 1ull << 18;  // per-path outputs staged per launch
+static __device__ __align__(256) char scratch[kScratchBytes];
+
+// Host buffers of the allocation-free entry point (static storage, no heap):
+// equal weights, the gathered losses of one VaR window, the refinement
+// histogram and the S histogram.  Guarded by g_mu like the entry point.
+static double g_na_weights[kMaxN];
+static float g_na_gather[kGatherCap];
+static unsigned g_na_hist[kHistBins];
+static unsigned long long g_na_shist[kMaxN + 1];
+
 Context g_ctx;
 std::mutex g_mu;
 
@@ -67,6 +86,14 @@ int ensure_init(int device) {
   HIP_TRY(hipEventCreate(&g_ctx.ev0));
   HIP_TRY(hipEventCreate(&g_ctx.ev1));
   HIP_TRY(hipEventCreate(&g_ctx.evs));
+  // the scratch array's address on this device (the device is current)
+  void* sp = nullptr;
+  HIP_TRY(hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)));
+  if (!sp) {
+    set_error("hipGetSymbolAddress returned no address for the scratch array");
+    return MCR_E_DEVICE;
+  }
+  g_ctx.scratch = sp;
   g_ctx.ready = true;
   return MCR_OK;
 }
@@ -282,46 +309,77 @@ int validate(const mcr_query* q) {
 struct Staged {
   Plan plan;
   DevBuffers d;
-  char* host;
+  char* host;  // pinned staging; nullptr in the allocation-free mode
 };
 
 // Uploads inputs, factors and prepares on the device.  Leaves the context
-// locked state untouched; caller holds g_mu.
-int stage_query(const mcr_query* q, Staged* st, bool want_var_windows) {
+// locked state untouched; caller holds g_mu.  With noalloc the device buffers
+// are carved from the scratch array and the inputs are copied straight from
+// the caller's arrays (no arena growth, no pinned staging).
+int stage_query(const mcr_query* q, Staged* st, bool want_var_windows, bool noalloc = false) {
   Plan& p = st->plan;
   p.n = q->n;
   MCR_TRY(choose_kernel(q, &p));
   p.r8 = ((q->n + kTileRows - 1) / kTileRows) * kTileRows;
   p.n_alpha = (q->flags & MCR_F_VAR) ? q->n_alpha : 0;
   p.per_path = q->S_out || q->loss_out;
-  p.chunk = p.per_path ? (size_t)std::min<uint64_t>((uint64_t)q->paths, kPerPathChunk) : 0;
+  p.chunk = p.per_path ? (size_t)std::min<uint64_t>((uint64_t)q->paths,
+                                                    noalloc ? kNoAllocChunk : kPerPathChunk)
+                       : 0;
   const size_t dev_bytes = layout(p, nullptr, &st->d);
-  MCR_TRY(reserve_arena(dev_bytes));
-  layout(p, (char*)g_ctx.arena, &st->d);
-
   const size_t n = (size_t)q->n;
-  const size_t in_bytes = align_up((n * n + 2 * n) * sizeof(double));
-  const size_t out_bytes = align_up(sizeof(DevInfo)) + align_up(sizeof(DevAccum)) +
-                           align_up(kHistBins * sizeof(unsigned)) +
-                           align_up((n + 1) * sizeof(unsigned long long));
-  MCR_TRY(reserve_host(in_bytes + out_bytes));
-  st->host = (char*)g_ctx.host;
-  double* hA = (double*)st->host;
-  double* hmu = hA + n * n;
-  double* hw = hmu + n;
-  for (size_t i = 0; i < n; ++i) {
-    std::memcpy(hA + i * n, q->sigma + i * (size_t)q->ld_sigma, n * sizeof(double));
-    hmu[i] = q->mu[i];
-    hw[i] = q->weights ? q->weights[i] : 1.0 / (double)n;
-  }
-  for (size_t i = 0; i < n; ++i)
-    if (!std::isfinite(hmu[i]) || !std::isfinite(hw[i])) {
-      set_error("mu or weights not finite at %zu", i);
-      return MCR_E_ARG;
+  if (noalloc) {
+    if (dev_bytes > kScratchBytes) {
+      set_error("query needs %zu bytes of device scratch, more than the %zu the "
+                "allocation-free entry point holds", dev_bytes, kScratchBytes);
+      return MCR_E_UNSUPPORTED;
     }
-  HIP_TRY(hipMemcpyAsync(st->d.A, hA, n * n * sizeof(double), hipMemcpyHostToDevice, g_ctx.stream));
-  HIP_TRY(hipMemcpyAsync(st->d.mu, hmu, n * sizeof(double), hipMemcpyHostToDevice, g_ctx.stream));
-  HIP_TRY(hipMemcpyAsync(st->d.w, hw, n * sizeof(double), hipMemcpyHostToDevice, g_ctx.stream));
+    layout(p, (char*)g_ctx.scratch, &st->d);
+    st->host = nullptr;
+    for (size_t i = 0; i < n; ++i) {
+      const double wi = q->weights ? q->weights[i] : 1.0 / (double)n;
+      if (!std::isfinite(q->mu[i]) || !std::isfinite(wi)) {
+        set_error("mu or weights not finite at %zu", i);
+        return MCR_E_ARG;
+      }
+      g_na_weights[i] = wi;
+    }
+    HIP_TRY(hipMemcpy2DAsync(st->d.A, n * sizeof(double), q->sigma,
+                             (size_t)q->ld_sigma * sizeof(double), n * sizeof(double), n,
+                             hipMemcpyHostToDevice, g_ctx.stream));
+    HIP_TRY(hipMemcpyAsync(st->d.mu, q->mu, n * sizeof(double), hipMemcpyHostToDevice,
+                           g_ctx.stream));
+    HIP_TRY(hipMemcpyAsync(st->d.w, g_na_weights, n * sizeof(double), hipMemcpyHostToDevice,
+                           g_ctx.stream));
+  } else {
+    MCR_TRY(reserve_arena(dev_bytes));
+    layout(p, (char*)g_ctx.arena, &st->d);
+    const size_t in_bytes = align_up((n * n + 2 * n) * sizeof(double));
+    const size_t out_bytes = align_up(sizeof(DevInfo)) + align_up(sizeof(DevAccum)) +
+                             align_up(kHistBins * sizeof(unsigned)) +
+                             align_up((n + 1) * sizeof(unsigned long long));
+    MCR_TRY(reserve_host(in_bytes + out_bytes));
+    st->host = (char*)g_ctx.host;
+    double* hA = (double*)st->host;
+    double* hmu = hA + n * n;
+    double* hw = hmu + n;
+    for (size_t i = 0; i < n; ++i) {
+      std::memcpy(hA + i * n, q->sigma + i * (size_t)q->ld_sigma, n * sizeof(double));
+      hmu[i] = q->mu[i];
+      hw[i] = q->weights ? q->weights[i] : 1.0 / (double)n;
+    }
+    for (size_t i = 0; i < n; ++i)
+      if (!std::isfinite(hmu[i]) || !std::isfinite(hw[i])) {
+        set_error("mu or weights not finite at %zu", i);
+        return MCR_E_ARG;
+      }
+    HIP_TRY(hipMemcpyAsync(st->d.A, hA, n * n * sizeof(double), hipMemcpyHostToDevice,
+                           g_ctx.stream));
+    HIP_TRY(hipMemcpyAsync(st->d.mu, hmu, n * sizeof(double), hipMemcpyHostToDevice,
+                           g_ctx.stream));
+    HIP_TRY(hipMemcpyAsync(st->d.w, hw, n * sizeof(double), hipMemcpyHostToDevice,
+                           g_ctx.stream));
+  }
 
   FactorArgs fa = {};
   fa.A = st->d.A;
@@ -357,6 +415,12 @@ int stage_query(const mcr_query* q, Staged* st, bool want_var_windows) {
 }
 
 int read_info(const Staged& st, DevInfo* info, DevAccum* acc) {
+  if (!st.host) {  // allocation-free mode: straight into the caller's structs
+    HIP_TRY(hipMemcpyAsync(info, st.d.info, sizeof(DevInfo), hipMemcpyDeviceToHost, g_ctx.stream));
+    HIP_TRY(hipMemcpyAsync(acc, st.d.acc, sizeof(DevAccum), hipMemcpyDeviceToHost, g_ctx.stream));
+    HIP_TRY(hipStreamSynchronize(g_ctx.stream));
+    return MCR_OK;
+  }
   char* out = st.host + align_up((size_t)st.plan.n * st.plan.n * sizeof(double) +
                                  2 * (size_t)st.plan.n * sizeof(double));
   DevInfo* hi = (DevInfo*)out;
@@ -446,21 +510,23 @@ int64_t fixed32_host(float v) { return (int64_t)((double)v * 4294967296.0); }
 // in-window losses in gather[0 .. min(G, cap))); returns 1 when resolved, 0
 // when the rank missed the window or the buffer overflowed, -1 on copy error.
 int resolve_quantile(int64_t M, const DevAccum& acc, int slot, const float* gather, unsigned cap,
-                     int64_t R, double* var, double* es, int64_t* tail) {
+                     int64_t R, double* var, double* es, int64_t* tail, float* hostbuf) {
   const int64_t G = (int64_t)acc.gathered[slot];
   const int64_t above = (int64_t)acc.above[slot];
   const int64_t below = M - above - G;
   if (G > (int64_t)cap || !(below < R && R <= below + G)) return 0;
-  std::vector<float> g((size_t)G);
+  // hostbuf (>= cap floats) serves the allocation-free mode; otherwise a heap copy
+  std::vector<float> heap(hostbuf ? 0 : (size_t)G);
+  float* g = hostbuf ? hostbuf : heap.data();
   if (G > 0) {
-    if (hipMemcpy(g.data(), gather, (size_t)G * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+    if (hipMemcpy(g, gather, (size_t)G * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
       return -1;
   }
   // The (idx+1)-th smallest in-window loss is the quantile; the values after
   // it in the partition are exactly the larger ones, and their fixed-point
   // sum does not depend on order, so a partition gives what a sort would.
   const int64_t idx = R - below - 1;
-  std::nth_element(g.begin(), g.begin() + idx, g.end());
+  std::nth_element(g, g + idx, g + G);
   int64_t t = acc.sum_above[slot];
   for (int64_t i = idx; i < G; ++i) t += fixed32_host(g[(size_t)i]);
   *var = (double)g[(size_t)idx];
@@ -472,7 +538,7 @@ int resolve_quantile(int64_t M, const DevAccum& acc, int slot, const float* gath
 int try_resolve(const mcr_query* q, const Staged& st, const DevAccum& acc, int slot, int64_t R,
                 double* var, double* es, int64_t* tail) {
   return resolve_quantile(q->paths, acc, slot, st.d.gather + (size_t)slot * kGatherCap, kGatherCap,
-                          R, var, es, tail);
+                          R, var, es, tail, st.host ? nullptr : g_na_gather);
 }
 
 int upload_windows(const Staged& st, const DevInfo& info) {
@@ -503,7 +569,8 @@ int refine_quantile(const mcr_query* q, const Staged& st, DevInfo info0, int al,
     a_lo = info0.lo[al];
     a_hi = info0.hi[al];
   }
-  std::vector<unsigned> hist(kHistBins);
+  std::vector<unsigned> heap_hist(st.host ? kHistBins : 0);
+  unsigned* hist = st.host ? heap_hist.data() : g_na_hist;
   for (int iter = 0; iter < 12; ++iter) {
     if (!(a_hi > a_lo)) a_hi = a_lo + 1e-6 * sd;
     DevInfo info = info0;
@@ -522,7 +589,7 @@ int refine_quantile(const mcr_query* q, const Staged& st, DevInfo info0, int al,
     DevAccum acc;
     DevInfo tmp;
     MCR_TRY(read_info(st, &tmp, &acc));
-    HIP_TRY(hipMemcpy(hist.data(), st.d.hist, kHistBins * sizeof(unsigned), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(hist, st.d.hist, kHistBins * sizeof(unsigned), hipMemcpyDeviceToHost));
     const int64_t hb = (int64_t)acc.hist_below;
     if (R <= hb) {
       a_hi = a_lo;
@@ -581,14 +648,23 @@ int refine_quantile(const mcr_query* q, const Staged& st, DevInfo info0, int al,
   return MCR_E_DEVICE;
 }
 
-int run_locked(const mcr_query* q, mcr_result* r) {
+int run_locked(const mcr_query* q, mcr_result* r, bool noalloc) {
   const auto t0 = std::chrono::steady_clock::now();
   MCR_TRY(validate(q));
-  MCR_TRY(ensure_init(g_ctx.device));
+  if (noalloc) {
+    // initialisation creates the stream, events and the scratch address:
+    // the allocation-free entry point only runs after mcr_init
+    if (!g_ctx.ready) {
+      set_error("mcr_run_noalloc needs mcr_init first (it creates nothing itself)");
+      return MCR_E_ARG;
+    }
+  } else {
+    MCR_TRY(ensure_init(g_ctx.device));
+  }
   HIP_TRY(hipSetDevice(g_ctx.device));
   std::memset(r, 0, sizeof(*r));
   Staged st;
-  MCR_TRY(stage_query(q, &st, true));
+  MCR_TRY(stage_query(q, &st, true, noalloc));
 
   PathArgs a = base_args(q, st);
   a.loss_mode = (q->flags & MCR_F_VAR) ? kLossBracket : kLossNone;
@@ -612,10 +688,11 @@ int run_locked(const mcr_query* q, mcr_result* r) {
   r->var_passes = (q->flags & MCR_F_VAR) ? 1 : 0;
 
   if (q->flags & MCR_F_SHIST) {
-    std::vector<unsigned long long> h((size_t)q->n + 1);
-    HIP_TRY(hipMemcpy(h.data(), st.d.s_hist, h.size() * sizeof(unsigned long long),
-                      hipMemcpyDeviceToHost));
-    for (size_t i = 0; i < h.size(); ++i) q->s_hist[i] = (int64_t)h[i];
+    const size_t nb = (size_t)q->n + 1;
+    std::vector<unsigned long long> heap(noalloc ? 0 : nb);
+    unsigned long long* h = noalloc ? g_na_shist : heap.data();
+    HIP_TRY(hipMemcpy(h, st.d.s_hist, nb * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < nb; ++i) q->s_hist[i] = (int64_t)h[i];
   }
 
   if (q->flags & MCR_F_VAR) {
@@ -1165,6 +1242,15 @@ int mcr_run(const mcr_query* q, mcr_result* r) {
   }
   std::lock_guard<std::mutex> lk(g_mu);
   return run_locked(q, r);
+}
+
+int mcr_run_noalloc(const mcr_query* q, mcr_result* r) {
+  if (!r) {
+    set_error("result is NULL");
+    return MCR_E_ARG;
+  }
+  std::lock_guard<std::mutex> lk(g_mu);
+  return run_locked(q, r, true);
 }
 
 int mcr_prepare(const mcr_query* q, mcr_model_export* out) {

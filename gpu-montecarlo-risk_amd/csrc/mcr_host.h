@@ -56,6 +56,10 @@ struct Context {
   void* host = nullptr;  // pinned staging
   size_t host_bytes = 0;
   uint64_t arena_gen = 0;  // bumped whenever the arena moves (invalidates graphs)
+  // Device address of the file-scope scratch array of mcr_api.cpp on this
+  // device (hipGetSymbolAddress at init); the array's own name is only a
+  // host-side handle and must never be passed to a kernel or a copy.
+  void* scratch = nullptr;
 };
 
 extern Context g_ctx;
@@ -69,9 +73,12 @@ double inv_norm_cdf(double p);
 int64_t rank_of(double alpha, int64_t M);
 int check_factor(const DevInfo& info);
 double group_cost(int T, int S);
+// hostbuf: nullptr, or a host buffer of >= cap floats used instead of a heap copy
 int resolve_quantile(int64_t M, const DevAccum& acc, int slot, const float* gather, unsigned cap,
-                     int64_t R, double* var, double* es, int64_t* tail);
-int run_locked(const mcr_query* q, mcr_result* r);
+                     int64_t R, double* var, double* es, int64_t* tail, float* hostbuf = nullptr);
+// noalloc: device buffers from the scratch array, host buffers static (no
+// hipMalloc, pinned or heap allocation); the library must be initialised
+int run_locked(const mcr_query* q, mcr_result* r, bool noalloc = false);
 void set_windows(const mcr_query* q, int n_alpha, FactorArgs* f);
 
 // Persistent batch classes: queries with n <= kSmallMaxN run the per-wave

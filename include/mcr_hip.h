@@ -131,6 +131,21 @@ int mcr_version(void);
 /* One risk query: factor, prepare, run all paths, reduce, statistics. */
 int mcr_run(const mcr_query *q, mcr_result *r);
 
+/* mcr_run without any allocation: no counterpart in the reference.  For
+ * callers that cannot allocate (tight serving loops, harnesses that forbid
+ * hipMalloc or host allocation while they run; the call synchronises the
+ * library's stream before it returns, so it is not itself recorded into a
+ * caller's HIP graph).  Same query, same kernels, same
+ * result as mcr_run, but every device buffer of the query comes from a
+ * 16 MiB array of static device memory that the library holds on the device
+ * until the process ends, and the host side uses static buffers: no
+ * hipMalloc, hipFree, pinned or heap allocation.  Needs mcr_init first;
+ * fails with MCR_E_UNSUPPORTED when the query's buffers exceed the array
+ * (they do not for n <= MCR_MAX_N; per-path outputs are staged 2^18 paths
+ * at a time).  Calls share the one array, so calls that overlap on one
+ * device race: they are serialised with mcr_run by the library's lock. */
+int mcr_run_noalloc(const mcr_query *q, mcr_result *r);
+
 /* GPU FP64 Cholesky of sigma with the jitter policy; L_out [n*n] row-major. */
 int mcr_factor(int n, const double *sigma, int ld_sigma, double *L_out, mcr_factor_info *info);
 

@@ -111,3 +111,13 @@ def test_server_footprint_and_servable_checks(mcr):
         mcr.server_footprint(dict(p, mu=mu))
     assert e.value.code == mcr.MCR_E_ARG
 
+
+def test_noalloc_entry_point_needs_init(mcr):
+    """mcr_run_noalloc creates nothing itself: before mcr_init (here no GPU is
+    ever initialised) it refuses the query without touching a device."""
+    q, keep = mcr._make_query(np.zeros(4), np.eye(4) * 1e-4, 0.02, 2, 64, 1)
+    r = mcr.Result()
+    code = mcr.lib().mcr_run_noalloc(ctypes.byref(q), ctypes.byref(r))
+    assert code == mcr.MCR_E_ARG
+    assert b"mcr_init" in mcr.lib().mcr_last_error()
+    del keep
