@@ -145,6 +145,56 @@ __device__ __forceinline__ void flush_tally(const PathArgs& a, const LaneTally& 
   }
 }
 
+// Workgroup form of flush_tally for kernels whose waves all tally paths: each
+// wave sums its lanes, the waves' sums meet in LDS, and wave 0 issues one
+// atomic per counter for the whole workgroup (a per-wave flush costs every
+// wave its own L2 atomics: at 8 waves per CU they were most of the launch's
+// HBM write traffic).  Every thread of the workgroup must call it (barrier).
+__device__ __forceinline__ void flush_tally_block(const PathArgs& a, const LaneTally& t) {
+  constexpr int kWaves = 16;              // workgroups of up to 1024 threads
+  constexpr int kU = 4 + kMaxAlpha;       // count, near, hist below/above, above[]
+  __shared__ uint32_t s_u[kWaves][kU];
+  __shared__ long long s_l[kWaves][kMaxAlpha];
+  const int lane = __lane_id(), wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  const bool bracket = a.loss_mode == kLossBracket, hist = a.loss_mode == kLossHist;
+  uint32_t u[kU];
+  long long l[kMaxAlpha];
+  u[0] = wave_sum_u32(t.count);
+  u[1] = wave_sum_u32(t.near);
+  u[2] = hist ? wave_sum_u32(t.hist_below) : 0u;
+  u[3] = hist ? wave_sum_u32(t.hist_above) : 0u;
+#pragma unroll
+  for (int al = 0; al < kMaxAlpha; ++al) {
+    const bool on = bracket && al < a.n_alpha;
+    u[4 + al] = on ? wave_sum_u32(t.above[al]) : 0u;
+    l[al] = on ? wave_sum_i64(t.sum_above[al]) : 0ll;
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < kU; ++k) s_u[wave][k] = u[k];
+#pragma unroll
+    for (int al = 0; al < kMaxAlpha; ++al) s_l[wave][al] = l[al];
+  }
+  __syncthreads();
+  if (wave == 0) {
+    unsigned long long* const dst[kU] = {&a.acc->count, &a.acc->near, &a.acc->hist_below,
+                                         &a.acc->hist_above, &a.acc->above[0], &a.acc->above[1],
+                                         &a.acc->above[2], &a.acc->above[3]};
+#pragma unroll
+    for (int k = 0; k < kU; ++k) {
+      const uint32_t v = wave_sum_u32(lane < nw ? s_u[lane][k] : 0u);
+      if (lane == 0 && v) atomicAdd(dst[k], (unsigned long long)v);
+    }
+#pragma unroll
+    for (int al = 0; al < kMaxAlpha; ++al) {
+      if (bracket && al < a.n_alpha) {
+        const long long v = wave_sum_i64(lane < nw ? s_l[lane][al] : 0ll);
+        if (lane == 0 && v) atomicAdd((unsigned long long*)&a.acc->sum_above[al], (unsigned long long)v);
+      }
+    }
+  }
+}
+
 __device__ __forceinline__ void zero_tally(LaneTally& t) {
   t.count = 0;
   t.near = 0;

@@ -128,7 +128,7 @@ __global__ __launch_bounds__(256) void k_mfma(PathArgs a) {
   const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x >> 6);
   mfma_wave_groups<NB, T>(a, (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6),
                           (a.npaths + 63) / 64, waves, t, want_hist ? sh_hist : nullptr);
-  flush_tally(a, t);
+  flush_tally_block(a, t);
   if (want_hist) {
     __syncthreads();
     for (int i = threadIdx.x; i <= n; i += blockDim.x)

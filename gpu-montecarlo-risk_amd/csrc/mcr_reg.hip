@@ -92,7 +92,7 @@ __global__ __launch_bounds__(256) void k_valu_reg(PathArgs a) {
     }
     tally_path(a, t, q, S, -g, near, want_hist ? sh_hist : nullptr);
   }
-  flush_tally(a, t);
+  flush_tally_block(a, t);
   if (want_hist) {
     __syncthreads();
     for (int i = threadIdx.x; i <= N; i += blockDim.x)
