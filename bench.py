@@ -166,7 +166,9 @@ def batch_leg(mcr, np, wl):
     res = {"workload": wl.CONFIG4["name"], "portfolios": len(ps), "total_paths": total,
            "n_min": min(len(p["mu"]) for p in ps), "n_max": max(len(p["mu"]) for p in ps),
            "autotune": {"candidates": tried, "chunk_paths": best.chunk_used,
-                        "order": "lpt" if best.order_used == 0 else "natural"},
+                        "order": "lpt" if best.order_used == 0 else "natural",
+                        "launch_streams": best.streams_used},
+           "launch_streams_used": o1.streams_used,
            "persistent_graph_device_ms": ms_b, "persistent_graph_wall_ms": wall_b,
            "persistent_paths_per_s": total / (ms_b * 1e-3),
            "graph_replayed": bool(o1.graph_replayed), "work_items": o1.work_items,

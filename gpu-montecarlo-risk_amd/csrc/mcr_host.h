@@ -44,6 +44,20 @@ struct Carver {
   }
 };
 
+// Streams the class launches of a batch (or a serving round) are forked over:
+// after the counters' clear kernel the forks wait for `fork`, each runs the
+// class kernels dealt to it, and the launching stream waits for every used
+// fork's `join` -- inside a captured graph these become the graph's edges.
+constexpr int kMaxForks = 4;
+struct ForkSet {
+  int n = 0;
+  hipStream_t s[kMaxForks] = {};
+  hipEvent_t fork = nullptr;
+  hipEvent_t join[kMaxForks] = {};
+};
+int create_forks(ForkSet* f, int n);  // default-priority non-blocking streams
+void destroy_forks(ForkSet* f);
+
 struct Context {
   bool ready = false;
   int device = 0;
@@ -56,6 +70,7 @@ struct Context {
   void* host = nullptr;  // pinned staging
   size_t host_bytes = 0;
   uint64_t arena_gen = 0;  // bumped whenever the arena moves (invalidates graphs)
+  ForkSet forks;           // class-launch streams of batches
   // Device address of the file-scope scratch array of mcr_api.cpp on this
   // device (hipGetSymbolAddress at init); the array's own name is only a
   // host-side handle and must never be passed to a kernel or a copy.
@@ -91,6 +106,7 @@ struct ClassRun {
   uint32_t off, count;  // item range
   int grid;             // resident workgroups of k
   int block;            // threads per workgroup of k
+  double cost;          // summed cost of the class's items (group_cost units)
 };
 BatchKernel class_kernel(int n);
 int class_block(int n);
@@ -98,8 +114,11 @@ int class_grid(BatchKernel k, int block);
 void order_by_class(std::vector<WorkItem>* items, const std::vector<BatchKernel>& qk,
                     const std::vector<int>& qb, const std::vector<double>& qcost, bool lpt,
                     std::vector<ClassRun>* runs);
+// nfork > 1 (and forks given): the class kernels run on min(nfork, forks->n)
+// concurrent streams, dealt longest-class-first to the least-loaded stream.
 int launch_runs(const std::vector<ClassRun>& runs, const PathArgs* qs, const WorkItem* items,
-                unsigned* next, int grid_override, hipStream_t stream);
+                unsigned* next, int grid_override, hipStream_t stream,
+                const ForkSet* forks = nullptr, int nfork = 1);
 void fill_statistics(int64_t M, const DevInfo& info, const DevAccum& acc, mcr_result* r);
 
 }  // namespace host

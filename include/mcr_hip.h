@@ -177,6 +177,9 @@ typedef struct mcr_batch_opts {
   int order_used;    /* out */
   int graph_replayed; /* out: 1 if an existing graph was replayed */
   int64_t work_items; /* out */
+  int streams;       /* concurrent streams the class launches are forked over (1..4);
+                        0 = the tuned default */
+  int streams_used;  /* out */
 } mcr_batch_opts;
 
 /* Runs nq independent queries.  Persistent mode: every query uses the bf16x3
@@ -186,9 +189,11 @@ typedef struct mcr_batch_opts {
  * MCR_KERNEL_MFMA_LDS above).  rs[i] receives query i's result. */
 int mcr_run_batch(const mcr_query *qs, int nq, mcr_result *rs, mcr_batch_opts *opts);
 
-/* Times candidate (chunk_paths, order) settings on this batch, keeps the
- * fastest as the library's tuned default (used when chunk_paths = 0) and
- * returns it in *best (chunk_used, order_used, ms_device).  n_candidates
+/* Times candidate settings on this batch -- work-item size x item order with
+ * the class launches on one stream, then 2-4 concurrent class-launch streams
+ * at the best of those -- keeps the fastest as the library's tuned default
+ * (used when chunk_paths = 0 / streams = 0) and returns it in *best
+ * (chunk_used, order_used, streams_used, ms_device).  n_candidates
  * (optional) receives how many settings were timed. */
 int mcr_autotune_batch(const mcr_query *qs, int nq, mcr_batch_opts *best, int *n_candidates);
 

@@ -100,9 +100,26 @@ def test_per_query_mode_agrees_statistically(mcr):
 def test_autotuner_picks_a_setting(mcr):
     ps = portfolios(7, sizes=(16, 48, 150), paths=[30000, 30000, 20000])
     best, tried = mcr.autotune_batch(ps)
-    assert tried == 8 and best.chunk_used in (512, 2048, 8192, 32768)
-    _, o = mcr.run_batch(ps, use_graph=False)  # chunk_paths = 0: the tuned default
+    # 4 item sizes x 2 orders on one stream, then 2, 3 and 4 launch streams
+    assert tried == 11 and best.chunk_used in (512, 2048, 8192, 32768)
+    assert 1 <= best.streams_used <= 4
+    _, o = mcr.run_batch(ps, use_graph=False)  # chunk_paths = 0, streams = 0: the tuned default
     assert o.chunk_used == best.chunk_used and o.order_used == best.order_used
+    assert o.streams_used == best.streams_used
+
+
+@pytest.mark.parametrize("streams", [2, 4])
+def test_forked_class_launches_equal_one_stream(mcr, streams):
+    """Class kernels forked over concurrent streams (eager and in a replayed
+    graph) give the results of one stream: items are keyed by (query, group)
+    and every reduction is an integer."""
+    ps = portfolios(9)
+    ref, _ = mcr.run_batch(ps, use_graph=False, chunk_paths=1024, streams=1)
+    for use_graph in (False, True, True):
+        res, o = mcr.run_batch(ps, use_graph=use_graph, chunk_paths=1024, streams=streams)
+        assert o.streams_used == streams
+        for x, y in zip(res, ref):
+            same(x, y)
 
 
 def test_batch_rejects_fp32_queries(mcr):
