@@ -16,7 +16,8 @@
 //           (factorisation skipped on a cache hit), k_prep_batch, H2D of the round's work
 //           items; event setup[p].  Overlaps the previous round's path kernel.
 //   paths:  wait setup[p]; k_batch_clear; one persistent launch per kernel class
-//           over the round's items (k_batch_small / k_batch_paths); events
+//           over the round's items (k_batch_small / k_batch_paths / k_batch_spec),
+//           the classes forked over nfork streams and joined back; events
 //           start[p], end[p].
 //   out:    wait end[p]; D2H of the records of queries whose last paths ran in
 //           round r; event out[p].
@@ -59,6 +60,11 @@ constexpr int64_t kDefaultBudget = 4ll << 30;
 constexpr int kDefaultCache = 256;
 constexpr int kDefaultActive = 1024;
 constexpr double kDefaultRoundMs = 4.0;
+// Class-launch streams per round.  One: forking a round over 3 streams made
+// the loop slower (3000-query stream at 800 QPS: p99 33 -> 63 ms; saturation
+// 1090 -> 880 queries/s), since setup, paths, output and the forks then share
+// the process's four hardware queues and setup work waits behind path kernels.
+constexpr int kDefaultForks = 1;
 constexpr uint32_t kMaxItems = 1u << 16;  // work items per round
 constexpr uint64_t kItemGroups = 32;      // 64-path groups per work item, at most
 constexpr int kMaxRuns = 64;              // kernel classes per round (at most 41 exist)
@@ -250,9 +256,11 @@ struct mcr_server {
   double round_ms = kDefaultRoundMs;
   int grid = 0;           // CUs: the round budget is grid x round_ms of device time
   int grid_override = 0;  // > 0: every class kernel launched with this many workgroups
+  int nfork = kDefaultForks;  // streams a round's class launches are forked over
 
   // device resources
   hipStream_t s_setup = nullptr, s_paths = nullptr, s_out = nullptr;
+  ForkSet forks;  // class-launch streams, joined back into s_paths
   hipEvent_t ev_setup[2] = {}, ev_start[2] = {}, ev_end[2] = {}, ev_out[2] = {};
   char* arena = nullptr;     // budget bytes: query regions and cached factors
   char* fixed = nullptr;     // slot table, round areas, histogram scratch
@@ -330,9 +338,11 @@ int mcr_server::init(int dev, const mcr_server_opts* o) {
   HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   grid = cus;
   grid_override = (o && o->grid > 0) ? o->grid : 0;
+  if (o && o->streams > 0) nfork = std::min(o->streams, kMaxForks);
   HIP_TRY(hipStreamCreateWithFlags(&s_setup, hipStreamNonBlocking));
   HIP_TRY(hipStreamCreateWithFlags(&s_paths, hipStreamNonBlocking));
   HIP_TRY(hipStreamCreateWithFlags(&s_out, hipStreamNonBlocking));
+  if (nfork > 1) MCR_TRY(create_forks(&forks, nfork));
   for (int p = 0; p < 2; ++p) {
     HIP_TRY(hipEventCreateWithFlags(&ev_setup[p], hipEventDisableTiming));
     HIP_TRY(hipEventCreate(&ev_start[p]));
@@ -391,6 +401,8 @@ void mcr_server::teardown() {
   (void)hipSetDevice(device);
   for (hipStream_t s : {s_setup, s_paths, s_out})
     if (s) (void)hipStreamSynchronize(s);
+  for (int j = 0; j < forks.n; ++j) (void)hipStreamSynchronize(forks.s[j]);
+  destroy_forks(&forks);
   for (int p = 0; p < 2; ++p) {
     for (hipEvent_t e : {ev_setup[p], ev_start[p], ev_end[p], ev_out[p]})
       if (e) (void)hipEventDestroy(e);
@@ -734,7 +746,7 @@ int mcr_server::enqueue_round(RoundRec& rr, const std::vector<Job*>& fresh,
   HIP_TRY(hipStreamWaitEvent(s_paths, ev_setup[p], 0));
   HIP_TRY(hipEventRecord(ev_start[p], s_paths));
   MCR_TRY(launch_runs(rr.runs, d_slots, (const WorkItem*)dr, (unsigned*)(dr + off_next),
-                      grid_override, s_paths));
+                      grid_override, s_paths, &forks, nfork));
   HIP_TRY(hipEventRecord(ev_end[p], s_paths));
 
   const size_t span = align_up(sizeof(DevInfo)) + sizeof(DevAccum);

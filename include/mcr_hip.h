@@ -218,6 +218,8 @@ typedef struct mcr_server_opts {
   int max_active;      /* queries admitted at once; 0 = 1024 */
   double round_ms;     /* target device time of one round; 0 = 4 ms */
   int grid;            /* persistent workgroups; 0 = one per CU */
+  int streams;         /* concurrent streams a round's class launches are forked
+                          over (1..4); 0 = 1 */
 } mcr_server_opts;
 
 typedef struct mcr_ticket_info {
