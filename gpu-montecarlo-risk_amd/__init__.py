@@ -321,7 +321,8 @@ class BatchOpts(ctypes.Structure):
                 ("ms_device", ctypes.c_double), ("ms_total", ctypes.c_double),
                 ("chunk_used", ctypes.c_int), ("order_used", ctypes.c_int),
                 ("graph_replayed", ctypes.c_int), ("work_items", ctypes.c_int64),
-                ("streams", ctypes.c_int), ("streams_used", ctypes.c_int)]
+                ("streams", ctypes.c_int), ("shapes", ctypes.c_int * 4),
+                ("streams_used", ctypes.c_int), ("big_shapes", ctypes.c_int * 4)]
 
 
 def _queries(portfolios):
@@ -341,12 +342,16 @@ def _queries(portfolios):
 
 
 def run_batch(portfolios, *, mode: int = BATCH_PERSISTENT, use_graph: bool = True,
-              chunk_paths: int = 0, order: int = ORDER_LPT, grid: int = 0, streams: int = 0):
-    """Independent queries in one batch (mcr_run_batch): (results, BatchOpts)."""
+              chunk_paths: int = 0, order: int = ORDER_LPT, grid: int = 0, streams: int = 0,
+              shapes=(0, 0, 0, 0)):
+    """Independent queries in one batch (mcr_run_batch): (results, BatchOpts).
+    shapes: per row-tile range, 0 = the tuned workgroup shape, s > 0 = shape s - 1."""
     qs, keep = _queries(portfolios)
     rs = (Result * len(portfolios))()
     o = BatchOpts(mode, int(use_graph), chunk_paths, order, grid)
     o.streams = int(streams)
+    for r, sh in enumerate(shapes):
+        o.shapes[r] = int(sh)
     _check(lib().mcr_run_batch(qs, len(portfolios), rs, ctypes.byref(o)))
     out = [_result(rs[i], len(portfolios[i].get("alphas", ()))) for i in range(len(portfolios))]
     del keep

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -751,12 +752,31 @@ void fill_statistics(int64_t M, const DevInfo& info, const DevAccum& acc, mcr_re
 // persistent launch of k_batch_paths over (query, 64-path group run) work
 // items, optionally captured once in a hipGraph and replayed.
 // ---------------------------------------------------------------------------
-BatchKernel class_kernel(int n) {
-  const int T = (n + 1 + 31) / 32;
-  return n <= kSmallMaxN ? batch_small_kernel(n, T) : batch_big_kernel(T);
+// Tuned workgroup shape per row-tile range of the LDS MFMA batch kernel
+// (index into the shapes of batch_big_kernel; 0 = the measured default).  The
+// batch autotuner writes them; batches and serving rounds read them.
+std::atomic<int> g_shape[kBigRanges] = {};
+
+void tuned_shapes(int out[kBigRanges]) {
+  for (int r = 0; r < kBigRanges; ++r) out[r] = g_shape[r].load(std::memory_order_relaxed);
 }
 
-int class_block(int n) { return n <= kSmallMaxN ? 256 : 64 * batch_big_waves((n + 1 + 31) / 32); }
+// Class of an n-asset query: the register-operand MFMA kernel of its (NB, T)
+// for n <= 127, else the LDS MFMA kernel of shape shapes[range of T] (the
+// tuned shapes when shapes is null).
+BatchKernel class_kernel(int n, const int* shapes) {
+  const int T = (n + 1 + 31) / 32;
+  if (n <= kSmallMaxN) return batch_small_kernel(n, T);
+  const int r = T <= 4 ? 0 : big_range(T);
+  return batch_big_kernel(T, shapes ? shapes[r] : g_shape[r].load(std::memory_order_relaxed));
+}
+
+int class_block(int n, const int* shapes) {
+  const int T = (n + 1 + 31) / 32;
+  if (n <= kSmallMaxN) return 256;
+  const int r = T <= 4 ? 0 : big_range(T);
+  return 64 * batch_big_waves(T, shapes ? shapes[r] : g_shape[r].load(std::memory_order_relaxed));
+}
 
 int class_grid(BatchKernel k, int block) {
   int cus = g_ctx.num_cus;
@@ -906,7 +926,8 @@ double group_cost(int T, int S) {
   return 4.0 * S + 1.5 * tiles;
 }
 
-int plan_batch(const mcr_query* qs, int nq, int chunk_paths, int order, BatchPlan* bp) {
+int plan_batch(const mcr_query* qs, int nq, int chunk_paths, int order, const int* shapes,
+               BatchPlan* bp) {
   bp->nq = nq;
   bp->T.resize(nq);
   bp->S.resize(nq);
@@ -942,8 +963,8 @@ int plan_batch(const mcr_query* qs, int nq, int chunk_paths, int order, BatchPla
   std::vector<double> qcost(nq);
   std::map<BatchKernel, uint64_t> class_groups;
   for (int i = 0; i < nq; ++i) {
-    qk[i] = class_kernel(qs[i].n);
-    qb[i] = class_block(qs[i].n);
+    qk[i] = class_kernel(qs[i].n, shapes);
+    qb[i] = class_block(qs[i].n, shapes);
     qcost[i] = group_cost(bp->T[i], bp->S[i]);
     class_groups[qk[i]] += ((uint64_t)qs[i].paths + 63) / 64;
   }
@@ -1084,9 +1105,10 @@ void fill_batch_inputs(const mcr_query* qs, const BatchPlan& bp, char* h, char* 
 }
 
 uint64_t batch_signature(const mcr_query* qs, const BatchPlan& bp, int grid, int chunk, int order,
-                         int streams) {
+                         int streams, const int* shapes) {
   uint64_t h = mix64(0, (uint64_t)(uintptr_t)g_ctx.arena);
   h = mix64(h, (uint64_t)streams);
+  for (int r = 0; r < kBigRanges; ++r) h = mix64(h, (uint64_t)shapes[r]);
   h = mix64(h, g_ctx.arena_gen);
   h = mix64(h, (uint64_t)(uintptr_t)g_ctx.host);  // the captured copy targets it
   h = mix64(h, (uint64_t)bp.nq);
@@ -1144,12 +1166,16 @@ int run_batch_locked(const mcr_query* qs, int nq, mcr_result* rs, mcr_batch_opts
   const int chunk = o->chunk_paths > 0 ? o->chunk_paths : g_tune.chunk_paths;
   const int order = o->chunk_paths > 0 ? o->order : g_tune.order;
   const int streams = std::min(o->streams > 0 ? o->streams : g_tune.streams, kMaxForks);
+  int shapes[kBigRanges];
+  tuned_shapes(shapes);
+  for (int r = 0; r < kBigRanges; ++r)
+    if (o->shapes[r] > 0) shapes[r] = std::min(o->shapes[r] - 1, batch_big_shapes(r) - 1);
   if (chunk % 64 != 0) {
     set_error("chunk_paths = %d is not a multiple of 64", chunk);
     return MCR_E_ARG;
   }
   BatchPlan bp;
-  MCR_TRY(plan_batch(qs, nq, chunk, order, &bp));
+  MCR_TRY(plan_batch(qs, nq, chunk, order, shapes, &bp));
   const int grid = o->grid;  // 0: each class kernel at its own occupancy
   MCR_TRY(reserve_arena(bp.total));
   MCR_TRY(reserve_host(bp.in_bytes + bp.out_bytes));
@@ -1161,7 +1187,7 @@ int run_batch_locked(const mcr_query* qs, int nq, mcr_result* rs, mcr_batch_opts
   HIP_TRY(hipEventRecord(g_ctx.ev0, g_ctx.stream));
   o->graph_replayed = 0;
   if (o->use_graph) {
-    const uint64_t sig = batch_signature(qs, bp, grid, chunk, order, streams);
+    const uint64_t sig = batch_signature(qs, bp, grid, chunk, order, streams, shapes);
     if (!g_graph.exec || g_graph.signature != sig) {
       drop_graph();
       HIP_TRY(hipStreamBeginCapture(g_ctx.stream, hipStreamCaptureModeThreadLocal));
@@ -1191,6 +1217,7 @@ int run_batch_locked(const mcr_query* qs, int nq, mcr_result* rs, mcr_batch_opts
   o->chunk_used = chunk;
   o->order_used = order;
   o->streams_used = streams;
+  for (int r = 0; r < kBigRanges; ++r) o->big_shapes[r] = shapes[r];
   o->work_items = (int64_t)bp.items.size();
 
   const DevInfo* infos = (const DevInfo*)hout;
@@ -1399,7 +1426,8 @@ int mcr_run_batch(const mcr_query* qs, int nq, mcr_result* rs, mcr_batch_opts* o
 int mcr_autotune_batch(const mcr_query* qs, int nq, mcr_batch_opts* best, int* n_candidates) {
   // Staged search on the caller's batch, device time by HIP events: work-item
   // size x item order with the class launches on one stream, then the number
-  // of concurrent class-launch streams at the best of those.
+  // of concurrent class-launch streams at the best of those, then the
+  // workgroup shape of each row-tile range the batch uses.
   static const int kChunks[] = {512, 2048, 8192, 32768};
   static const int kOrders[] = {MCR_ORDER_LPT, MCR_ORDER_NATURAL};
   static const int kStreams[] = {2, 3, 4};
@@ -1408,12 +1436,15 @@ int mcr_autotune_batch(const mcr_query* qs, int nq, mcr_batch_opts* best, int* n
   double best_ms = 1e300;
   mcr_batch_opts pick = {};
   int tried = 0;
+  int cur[kBigRanges];
+  tuned_shapes(cur);
   auto trial = [&](int chunk, int order, int streams) -> int {
     mcr_batch_opts o = {};
     o.mode = MCR_BATCH_PERSISTENT;
     o.chunk_paths = chunk;
     o.order = order;
     o.streams = streams;
+    for (int r = 0; r < kBigRanges; ++r) o.shapes[r] = cur[r] + 1;
     MCR_TRY(run_batch_locked(qs, nq, rs.data(), &o));
     ++tried;
     if (o.ms_device < best_ms) {
@@ -1426,6 +1457,24 @@ int mcr_autotune_batch(const mcr_query* qs, int nq, mcr_batch_opts* best, int* n
     for (int od : kOrders) MCR_TRY(trial(c, od, 1));
   const int c0 = pick.chunk_used, o0 = pick.order_used;
   for (int st : kStreams) MCR_TRY(trial(c0, o0, st));
+  // then, for each row-tile range the batch uses, its alternative workgroup
+  // shapes (one range at a time, the others at their best so far)
+  bool in_range[kBigRanges] = {};
+  for (int i = 0; i < nq; ++i) {
+    const int T = (qs[i].n + 1 + 31) / 32;
+    if (qs[i].n > kSmallMaxN && T > 4) in_range[big_range(T)] = true;
+  }
+  for (int r = 0; r < kBigRanges; ++r) {
+    if (!in_range[r]) continue;
+    const int keep = pick.big_shapes[r];
+    for (int sh = 0; sh < batch_big_shapes(r); ++sh) {
+      if (sh == keep) continue;
+      for (int q = 0; q < kBigRanges; ++q) cur[q] = pick.big_shapes[q];
+      cur[r] = sh;
+      MCR_TRY(trial(pick.chunk_used, pick.order_used, pick.streams_used));
+    }
+  }
+  for (int r = 0; r < kBigRanges; ++r) g_shape[r].store(pick.big_shapes[r]);
   g_tune.chunk_paths = pick.chunk_used;
   g_tune.order = pick.order_used;
   g_tune.streams = pick.streams_used;

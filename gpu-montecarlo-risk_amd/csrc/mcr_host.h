@@ -108,8 +108,9 @@ struct ClassRun {
   int block;            // threads per workgroup of k
   double cost;          // summed cost of the class's items (group_cost units)
 };
-BatchKernel class_kernel(int n);
-int class_block(int n);
+// shapes: per row-tile range, the LDS MFMA kernel shape; nullptr = the tuned ones
+BatchKernel class_kernel(int n, const int* shapes = nullptr);
+int class_block(int n, const int* shapes = nullptr);
 int class_grid(BatchKernel k, int block);
 void order_by_class(std::vector<WorkItem>* items, const std::vector<BatchKernel>& qk,
                     const std::vector<int>& qb, const std::vector<double>& qcost, bool lpt,

@@ -135,9 +135,15 @@ struct BatchArgs {
 // per workgroup and row-tile slots per wave).
 typedef void (*BatchKernel)(BatchArgs);
 BatchKernel batch_small_kernel(int n, int T);  // n <= 127, 256 threads
-BatchKernel batch_big_kernel(int T);           // T <= 24, 64 * batch_big_waves(T) threads
 int big_waves(int T);                          // waves per workgroup, single-query LDS MFMA kernel
-int batch_big_waves(int T);                    // waves per workgroup, batch LDS MFMA kernel
+// Batch LDS MFMA kernel shapes: row-tile ranges T = 5-8, 9-12, 13-16, 17-24,
+// each with kMaxBigShapes interchangeable shapes (shape 0 the default).
+constexpr int kBigRanges = 4;
+constexpr int kMaxBigShapes = 2;
+int big_range(int T);                          // range of T (T >= 5)
+int batch_big_shapes(int range);               // shapes of a range
+BatchKernel batch_big_kernel(int T, int shape);  // T <= 24, 64 * batch_big_waves threads
+int batch_big_waves(int T, int shape);         // waves per workgroup of that kernel
 int resident_blocks(const void* kernel, int block);  // workgroups per CU (occupancy)
 hipError_t launch_batch_clear(unsigned* next, int count, hipStream_t stream);
 hipError_t launch_batch_kernel(BatchKernel k, const BatchArgs& b, int grid, int block,

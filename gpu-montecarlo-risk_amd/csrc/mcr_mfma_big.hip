@@ -550,14 +550,34 @@ hipError_t launch_batch_kernel(BatchKernel k, const BatchArgs& b, int grid, int 
 // loses below 13 tiles (12 + 4 waves of 1 tile: +3 % at n = 300; 8 + 4 of 2:
 // +5 %; 4 + 4 of 3: +9 %).
 int big_waves(int T) { return T <= 12 ? 4 : 8 + kProducers; }
-int batch_big_waves(int T) { return T <= 12 ? 4 : (T <= 16 ? 8 : 8 + kProducers); }
 
-BatchKernel batch_big_kernel(int T) {
+// Batch shapes per row-tile range, the measured default first, then the
+// alternatives the batch autotuner may choose (every shape gives the same
+// bits: each accumulator receives the same MFMA sequence whichever wave holds
+// it).  Range 0: T = 5-8 (n = 128-255), 1: T = 9-12, 2: T = 13-16, 3: T = 17-24.
+struct BigShape {
+  BatchKernel k;
+  int waves;
+};
+static const BigShape kBigShapes[kBigRanges][kMaxBigShapes] = {
+    {{k_batch_paths<2, 4, false>, 4}, {k_batch_paths<1, 8, true>, 8}},
+    {{k_batch_paths<3, 4, false>, 4}, {k_batch_paths<2, 8, false>, 8}},
+    {{k_batch_paths<2, 8, false>, 8}, {k_batch_spec<8, 2, 2>, 8 + kProducers}},
+    {{k_batch_spec<8, 3, 2>, 8 + kProducers}, {k_batch_spec<12, 2, 2>, 12 + kProducers}},
+};
+
+int big_range(int T) { return T <= 8 ? 0 : (T <= 12 ? 1 : (T <= 16 ? 2 : 3)); }
+int batch_big_shapes(int range) { return range >= 0 && range < kBigRanges ? kMaxBigShapes : 0; }
+
+static int clamp_shape(int shape) { return shape >= 0 && shape < kMaxBigShapes ? shape : 0; }
+
+int batch_big_waves(int T, int shape) {
+  return T <= 4 ? 4 : kBigShapes[big_range(T)][clamp_shape(shape)].waves;
+}
+
+BatchKernel batch_big_kernel(int T, int shape) {
   if (T <= 4) return k_batch_paths<1, 4, true>;
-  if (T <= 8) return k_batch_paths<2, 4, false>;
-  if (T <= 12) return k_batch_paths<3, 4, false>;
-  if (T <= 16) return k_batch_paths<2, 8, false>;
-  return T <= 24 ? k_batch_spec<8, 3, 2> : nullptr;
+  return T <= 24 ? kBigShapes[big_range(T)][clamp_shape(shape)].k : nullptr;
 }
 
 PathKernel mfma_big_kernel(int T) {

@@ -179,7 +179,11 @@ typedef struct mcr_batch_opts {
   int64_t work_items; /* out */
   int streams;       /* concurrent streams the class launches are forked over (1..4);
                         0 = the tuned default */
+  int shapes[4];     /* workgroup shape of the LDS MFMA kernel per row-tile range
+                        (n 128-255, 256-383, 384-511, 512-608): 0 = the tuned shape,
+                        s > 0 = shape s - 1 (all shapes give identical results) */
   int streams_used;  /* out */
+  int big_shapes[4]; /* out: shape used per range (0 = the measured default) */
 } mcr_batch_opts;
 
 /* Runs nq independent queries.  Persistent mode: every query uses the bf16x3
@@ -191,10 +195,12 @@ int mcr_run_batch(const mcr_query *qs, int nq, mcr_result *rs, mcr_batch_opts *o
 
 /* Times candidate settings on this batch -- work-item size x item order with
  * the class launches on one stream, then 2-4 concurrent class-launch streams
- * at the best of those -- keeps the fastest as the library's tuned default
- * (used when chunk_paths = 0 / streams = 0) and returns it in *best
- * (chunk_used, order_used, streams_used, ms_device).  n_candidates
- * (optional) receives how many settings were timed. */
+ * at the best of those, then the alternative workgroup shapes of each
+ * row-tile range the batch uses -- keeps the fastest as the library's tuned
+ * default (chunk and order used when chunk_paths = 0, streams when
+ * streams = 0, shapes always, also by the serving loop) and returns it in
+ * *best (chunk_used, order_used, streams_used, big_shapes, ms_device).
+ * n_candidates (optional) receives how many settings were timed. */
 int mcr_autotune_batch(const mcr_query *qs, int nq, mcr_batch_opts *best, int *n_candidates);
 
 /* ---- serving loop (continuous batching) ----

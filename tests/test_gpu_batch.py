@@ -100,12 +100,29 @@ def test_per_query_mode_agrees_statistically(mcr):
 def test_autotuner_picks_a_setting(mcr):
     ps = portfolios(7, sizes=(16, 48, 150), paths=[30000, 30000, 20000])
     best, tried = mcr.autotune_batch(ps)
-    # 4 item sizes x 2 orders on one stream, then 2, 3 and 4 launch streams
-    assert tried == 11 and best.chunk_used in (512, 2048, 8192, 32768)
-    assert 1 <= best.streams_used <= 4
+    # 4 item sizes x 2 orders on one stream, then 2, 3 and 4 launch streams,
+    # then the other workgroup shape of the one row-tile range used (n = 150)
+    assert tried == 12 and best.chunk_used in (512, 2048, 8192, 32768)
+    assert 1 <= best.streams_used <= 4 and best.big_shapes[0] in (0, 1)
     _, o = mcr.run_batch(ps, use_graph=False)  # chunk_paths = 0, streams = 0: the tuned default
     assert o.chunk_used == best.chunk_used and o.order_used == best.order_used
     assert o.streams_used == best.streams_used
+    assert list(o.big_shapes) == list(best.big_shapes)
+
+
+def test_every_workgroup_shape_gives_the_same_results(mcr):
+    """The LDS MFMA batch kernel's alternative workgroup shapes (the
+    autotuner's choices) give bit-identical results: each accumulator receives
+    the same MFMA sequence whichever wave holds it."""
+    ps = portfolios(10, sizes=(130, 250, 260, 380, 390, 500, 520, 608),
+                    paths=[9000, 7000, 6000, 5000, 4000, 3000, 3000, 2000])
+    ref, o = mcr.run_batch(ps, use_graph=False, chunk_paths=512, shapes=(1, 1, 1, 1))
+    assert list(o.big_shapes) == [0, 0, 0, 0]
+    alt, o2 = mcr.run_batch(ps, use_graph=True, chunk_paths=512, shapes=(2, 2, 2, 2))
+    assert list(o2.big_shapes) == [1, 1, 1, 1]
+    for p, x, y in zip(ps, alt, ref):
+        same(x, y)
+        same(x, alone(mcr, p))
 
 
 @pytest.mark.parametrize("streams", [2, 4])
