@@ -26,12 +26,16 @@ PEAK_BF16_TFLOPS = 2500.0   # MI355X published dense bf16 MFMA
 # HBM bytes per launch of the headline path kernel, from two rocprofv3 --pmc
 # runs of this bench (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 KiB, the factors
 # read off a 256 MiB torch.neg calibration kernel in the same runs).
-TRAFFIC_BYTES = {"mfma": (178.50 * 2.000 + 1410.66) * 1024}
-# VALU figures of the headline path kernel from rocprofv3 --pmc runs of this
-# bench (SQ_ACTIVE_INST_VALU, SQ_VALU_MFMA_BUSY_CYCLES, SQ_INSTS_VALU over
-# GRBM_GUI_ACTIVE): the unit that binds it is the VALU issue of normal
-# generation, not the matrix cores.
-VALU_PROFILE = {"mfma": None}
+TRAFFIC_BYTES = {"mfma": (191.5 * 2.000 + 834.7) * 1024}
+# VALU figures of the headline path kernel from a rocprofv3 --pmc run of this
+# bench (SQ_INSTS_VALU, SQ_ACTIVE_INST_VALU2, SQ_VALU_MFMA_BUSY_CYCLES over
+# GRBM_GUI_ACTIVE, 1024 SIMDs): the unit that binds it is the VALU issue of
+# normal generation (Philox4x32-10, Box-Muller), not the matrix cores.
+VALU_PROFILE = {"mfma": {"valu_lane_instr_per_path": 2509,
+                         "valu_instr_per_simd_quad_cycle": 1.19,
+                         "valu_issue_quad_cycle_frac": 0.914,
+                         "mfma_busy_frac": 0.137,
+                         "source": "profiles/k_mfma_n50_100M.md"}}
 
 
 def parse():
@@ -167,7 +171,8 @@ def batch_leg(mcr, np, wl):
            "n_min": min(len(p["mu"]) for p in ps), "n_max": max(len(p["mu"]) for p in ps),
            "autotune": {"candidates": tried, "chunk_paths": best.chunk_used,
                         "order": "lpt" if best.order_used == 0 else "natural",
-                        "launch_streams": best.streams_used},
+                        "launch_streams": best.streams_used,
+                        "lds_shape_per_range": list(best.big_shapes)},
            "launch_streams_used": o1.streams_used,
            "persistent_graph_device_ms": ms_b, "persistent_graph_wall_ms": wall_b,
            "persistent_paths_per_s": total / (ms_b * 1e-3),
