@@ -38,7 +38,7 @@ MCR_MAX_ALPHA = 4
 MCR_MAX_N = 608
 PREC_FP32, PREC_BF16X3, PREC_AUTO = 0, 1, 2
 KERNEL_AUTO, KERNEL_VALU_REG, KERNEL_VALU_TILE, KERNEL_MFMA, KERNEL_MFMA_LDS = 0, 1, 2, 3, 4
-F_VAR, F_NEAR, F_SHIST, F_TEST_NARROW = 0x1, 0x2, 0x4, 0x100
+F_VAR, F_NEAR, F_SHIST, F_TEST_NARROW, F_TEST_WIDE = 0x1, 0x2, 0x4, 0x100, 0x200
 KERNEL_NAMES = {KERNEL_VALU_REG: "valu_reg", KERNEL_VALU_TILE: "valu_tile", KERNEL_MFMA: "mfma",
                 KERNEL_MFMA_LDS: "mfma_lds"}
 

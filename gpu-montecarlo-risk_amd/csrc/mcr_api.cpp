@@ -728,7 +728,9 @@ void set_windows(const mcr_query* q, int n_alpha, FactorArgs* f) {
     f->za[al] = za;
     f->ha[al] = (q->flags & MCR_F_TEST_NARROW)
                     ? 0.0
-                    : 6.0 * std::sqrt(av * (1.0 - av) / (double)q->paths) / dens + 1e-4;
+                    : (q->flags & MCR_F_TEST_WIDE)
+                          ? 1e6
+                          : 6.0 * std::sqrt(av * (1.0 - av) / (double)q->paths) / dens + 1e-4;
   }
 }
 

@@ -60,6 +60,9 @@ extern "C" {
 #define MCR_F_SHIST 0x4 /* histogram of S (n+1 bins) into s_hist */
 /* test hook: zero-width VaR windows, forcing the quantile refinement passes */
 #define MCR_F_TEST_NARROW 0x100
+/* test hook: VaR windows that hold every loss (every path appends its loss);
+   needs paths <= 2^18 to stay one pass */
+#define MCR_F_TEST_WIDE 0x200
 
 typedef struct mcr_query {
   int n;                /* assets, 1 <= n <= MCR_MAX_N */

@@ -57,6 +57,39 @@ def test_var_refinement_path(mcr, orc, kernel):
         assert r.var[i] == var and r.tail_sum_fixed[i] == fixed and r.es[i] == es
 
 
+@pytest.mark.parametrize("kernel", [1, 2])
+def test_var_windows_holding_every_loss(mcr, orc, kernel):
+    """Windows wide enough to hold every loss (every path appends its loss to
+    the gather buffer): the order statistics, tail sums and ES stay exact, in
+    one pass."""
+    n, M, seed = 24, 200_000, 7
+    mu, sigma = factor_like(n, 3, 26)
+    alphas = (0.5, 0.99)
+    r = mcr.run(mu, sigma, 0.02, 4, M, seed, kernel=kernel, alphas=alphas,
+                extra_flags=mcr.F_TEST_WIDE)
+    ref = oracle_quantiles(orc, mcr, mu, sigma, 0.02, None, seed, M, alphas)
+    assert r.var_passes == 1
+    for i, (var, fixed, es) in enumerate(ref):
+        assert r.var[i] == var and r.tail_sum_fixed[i] == fixed and r.es[i] == es
+
+
+@pytest.mark.parametrize("n", [24, 100])
+def test_mfma_var_independent_of_window_width(mcr, n):
+    """The register-operand MFMA kernel gives the same order statistics, tail
+    sums and ES whether a few thousand losses or all of them fall in the
+    windows."""
+    M, seed = 200_000, 8
+    mu, sigma = factor_like(n, 3, 27)
+    alphas = (0.5, 0.99, 0.999)
+    a = mcr.run(mu, sigma, 0.02, 4, M, seed, precision=mcr.PREC_BF16X3, alphas=alphas)
+    b = mcr.run(mu, sigma, 0.02, 4, M, seed, precision=mcr.PREC_BF16X3, alphas=alphas,
+                extra_flags=mcr.F_TEST_WIDE)
+    assert a.kernel == b.kernel == "mfma" and b.var_passes == 1
+    assert list(a.var) == list(b.var)
+    assert list(a.tail_sum_fixed) == list(b.tail_sum_fixed)
+    assert list(a.es) == list(b.es)
+
+
 @pytest.mark.parametrize("M", [1, 2, 10, 100])
 def test_var_tiny_path_counts(mcr, orc, M):
     mu, sigma = factor_like(6, 2, 23)
