@@ -541,7 +541,11 @@ hipError_t launch_batch_kernel(BatchKernel k, const BatchArgs& b, int grid, int 
 // queries run the split form: 8 multiplying waves of 2 (T <= 16) or 3 tiles
 // beside 4 generating waves (n = 450: 6.0 ms against 6.4 ms for eight waves of
 // 2 tiles; n = 600: 14.85 ms against 15.0 ms for 12 multiplying waves of 2
-// tiles and 16.8 ms for eight waves of 3 tiles that all generate).  Batches
+// tiles and 16.8 ms for eight waves of 3 tiles that all generate).  The 3-tile
+// split form runs 4 K steps per phase: half the workgroup barriers of 2-step
+// phases, 0.5-1.7 % faster for n = 400-600 in interleaved timings (6- and
+// 8-step phases no faster; the 2-tile form gained at n = 450-500 and lost at
+// n = 400, so it keeps 2-step phases).  Batches
 // keep eight waves of 2 tiles at four waves per SIMD for 13-16 tiles (a
 // persistent workgroup there runs many short work items, and two per CU cover
 // each other's item changes; the 8 + 4 split form, one workgroup per CU, made
@@ -563,7 +567,7 @@ static const BigShape kBigShapes[kBigRanges][kMaxBigShapes] = {
     {{k_batch_paths<2, 4, false>, 4}, {k_batch_paths<1, 8, true>, 8}},
     {{k_batch_paths<3, 4, false>, 4}, {k_batch_paths<2, 8, false>, 8}},
     {{k_batch_paths<2, 8, false>, 8}, {k_batch_spec<8, 2, 2>, 8 + kProducers}},
-    {{k_batch_spec<8, 3, 2>, 8 + kProducers}, {k_batch_spec<12, 2, 2>, 12 + kProducers}},
+    {{k_batch_spec<8, 3, 4>, 8 + kProducers}, {k_batch_spec<12, 2, 2>, 12 + kProducers}},
 };
 
 int big_range(int T) { return T <= 8 ? 0 : (T <= 12 ? 1 : (T <= 16 ? 2 : 3)); }
@@ -585,7 +589,7 @@ PathKernel mfma_big_kernel(int T) {
   if (T <= 8) return k_mfma_big<2, 4, false>;
   if (T <= 12) return k_mfma_big<3, 4, false>;
   if (T <= 16) return k_mfma_spec<8, 2, 2>;
-  return T <= 24 ? k_mfma_spec<8, 3, 2> : nullptr;
+  return T <= 24 ? k_mfma_spec<8, 3, 4> : nullptr;
 }
 
 }  // namespace mcr
