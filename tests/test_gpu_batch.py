@@ -51,6 +51,18 @@ def test_batch_equals_alone(mcr):
         assert r.kernel == ("mfma" if len(p["mu"]) <= 127 else "mfma_lds")
 
 
+def test_batch_equals_alone_at_class_boundaries(mcr):
+    """Sizes on both sides of every kernel-class boundary (the register-operand
+    MFMA kernel up to n = 127, the LDS row-tile ranges 5-8, 9-12, 13-16 and
+    17-24 tiles, the 608 limit) batch to the results of the queries alone."""
+    sizes = (127, 128, 255, 256, 383, 384, 511, 512, 607, 608)
+    ps = portfolios(11, sizes=sizes, paths=[3000 + 17 * i for i in range(len(sizes))])
+    for use_graph in (False, True):
+        res, o = mcr.run_batch(ps, use_graph=use_graph, chunk_paths=512)
+        for p, r in zip(ps, res):
+            same(r, alone(mcr, p))
+
+
 def test_graph_replay_uses_new_inputs(mcr):
     a = portfolios(2)
     b = portfolios(3, paths=[p["paths"] for p in a])  # same shapes, new data and seeds
