@@ -111,11 +111,11 @@ def cpu_baseline(mcr, orc, W, mu, sigma, gpu_full, serial_paths):
 
 
 def plain_leg(mcr, orc, wl):
-    """Config 1 (n = 20, 1M paths, k = 5): the serial C path (oracle, 1 core)
+    """The plain n = 20 workload (1M paths, k = 5): the serial C path (oracle, 1 core)
     against the GPU fp32 kernel on the identical stream, counts compared."""
     import numpy as np
 
-    C = wl.CONFIG1
+    C = wl.PLAIN_N20
     mu, sigma = wl.model_of(mcr, C)
     runs = [mcr.run(mu, sigma, C["x"], C["k"], C["paths"], C["path_seed"], precision=mcr.PREC_FP32,
                     near=True) for _ in range(4)]
@@ -151,7 +151,7 @@ def same_result(a, b):
 
 
 def batch_leg(mcr, np, wl):
-    """Config 4: persistent class launches in one hipGraph vs one query at a
+    """256 uneven portfolios: persistent class launches in one hipGraph vs one query at a
     time; every batched result must equal the same query run alone."""
     ps = wl.batch_portfolios(mcr)
     total = sum(p["paths"] for p in ps)
@@ -168,7 +168,7 @@ def batch_leg(mcr, np, wl):
         raise SystemExit(f"batch leg: {len(mism)} batched results differ from the query alone "
                          f"(first: {mism[:5]})")
     ms_b, wall_b = float(np.median(dev)), float(np.median(wall))
-    res = {"workload": wl.CONFIG4["name"], "portfolios": len(ps), "total_paths": total,
+    res = {"workload": wl.BATCH_256["name"], "portfolios": len(ps), "total_paths": total,
            "n_min": min(len(p["mu"]) for p in ps), "n_max": max(len(p["mu"]) for p in ps),
            "autotune": {"candidates": tried, "chunk_paths": best.chunk_used,
                         "order": "lpt" if best.order_used == 0 else "natural",
@@ -189,10 +189,10 @@ def batch_leg(mcr, np, wl):
 
 
 def serve_leg(mcr, np, wl, count, rates):
-    """Config 5: open-loop Poisson arrivals into the serving loop at each
+    """The Poisson query stream: open-loop arrivals into the serving loop at each
     offered rate (the same queries, arrival gaps scaled); latency is submit ->
     result ready as the server records it."""
-    S = wl.CONFIG5
+    S = wl.SERVE_200QPS
     qs, gaps, n_distinct = wl.serve_queries(mcr, count)
     warm, _, _ = wl.serve_queries(mcr, 8, S["seed"] + 1)
     paths = sum(q["paths"] for q in qs)
@@ -240,8 +240,8 @@ def serve_leg(mcr, np, wl, count, rates):
 
 
 def large_leg(mcr, np, wl):
-    """n = 600 (config 3): plain fp32 tile kernel vs bf16x3 MFMA, same stream."""
-    G = wl.CONFIG3
+    """n = 600, 1e7 paths: plain fp32 tile kernel vs bf16x3 MFMA, same stream."""
+    G = wl.LARGE_N600
     mu, sigma = wl.model_of(mcr, G)
     res = {"workload": G["name"], "n_assets": G["n"], "paths": G["paths"],
            "cond_sigma": float(np.linalg.cond(sigma))}
@@ -279,7 +279,7 @@ def accuracy_check(mcr, np, orc, wl):
     one (the FP64 oracle's n^2 mat-vec per path bounds that sample)."""
     threads, _ = cpu_share()
     out = {}
-    for name, C, M in (("n50", wl.CONFIG2, wl.CONFIG2["paths"]), ("n600", wl.CONFIG3, 1_000_000)):
+    for name, C, M in (("n50", wl.FUSED_N50, wl.FUSED_N50["paths"]), ("n600", wl.LARGE_N600, 1_000_000)):
         n = C["n"]
         mu, sigma = wl.model_of(mcr, C)
         r = mcr.run(mu, sigma, C["x"], C["k"], M, C["path_seed"], precision=mcr.PREC_BF16X3,
@@ -324,7 +324,7 @@ def main():
         torch.cuda.synchronize()
         print(f"# calibration kernel (neg): {args.copy_mb} MiB read + {args.copy_mb} MiB written",
               file=sys.stderr, flush=True)
-    W = wl.CONFIG2
+    W = wl.FUSED_N50
     n = W["n"]
     mu, sigma = wl.model_of(mcr, W)
     prec = {"auto": mcr.PREC_AUTO, "fp32": mcr.PREC_FP32, "bf16x3": mcr.PREC_BF16X3}[args.precision]

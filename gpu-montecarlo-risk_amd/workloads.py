@@ -15,31 +15,32 @@ from concurrent.futures import ThreadPoolExecutor
 import numpy as np
 
 # Plain method, small: n = 20 assets, 3 factors over 1000 days, 1M paths, k = 5.
-CONFIG1 = {"name": "plain_n20_1M", "n": 20, "factors": 3, "days": 1000, "paths": 1_000_000,
-           "k": 5, "x": 0.02, "data_seed": 1001, "path_seed": 20251201}
+PLAIN_N20 = {"name": "plain_n20_1M", "n": 20, "factors": 3, "days": 1000, "paths": 1_000_000,
+             "k": 5, "x": 0.02, "data_seed": 1001, "path_seed": 20251201}
 # Fused kernel at scale: n = 50, 5 factors over 2000 days, 100M paths, k = 10, VaR99.
-CONFIG2 = {"name": "fused_n50_100M", "n": 50, "factors": 5, "days": 2000, "paths": 100_000_000,
-           "k": 10, "x": 0.02, "alpha": 0.99, "data_seed": 1002, "path_seed": 20251202}
+FUSED_N50 = {"name": "fused_n50_100M", "n": 50, "factors": 5, "days": 2000, "paths": 100_000_000,
+             "k": 10, "x": 0.02, "alpha": 0.99, "data_seed": 1002, "path_seed": 20251202}
 # Large n: n = 600, 20 factors over 2500 days (cond ~1e4), 10M paths, VaR99 / VaR99.9.
-CONFIG3 = {"name": "large_n600_10M", "n": 600, "factors": 20, "days": 2500,
-           "paths": 10_000_000, "k": 10, "x": 0.02, "alphas": (0.99, 0.999),
-           "data_seed": 1003, "path_seed": 20251203}
+LARGE_N600 = {"name": "large_n600_10M", "n": 600, "factors": 20, "days": 2500,
+              "paths": 10_000_000, "k": 10, "x": 0.02, "alphas": (0.99, 0.999),
+              "data_seed": 1003, "path_seed": 20251203}
 # 256 uneven portfolios in one batch.
-CONFIG4 = {"name": "batch_256_uneven", "count": 256, "n_range": (5, 600),
-           "paths_range": (1e5, 1e7), "x_range": (0.01, 0.04), "alpha": 0.99, "seed": 1004}
-# Poisson query stream at a fixed rate, sizes as config 4, 30 % covariance reuse.
-CONFIG5 = {"name": "serve_poisson_200qps", "qps": 200.0, "reuse": 0.3, "n_range": (5, 600),
-           "paths_range": (1e5, 1e7), "x_range": (0.01, 0.04), "alpha": 0.99, "seed": 1005}
+BATCH_256 = {"name": "batch_256_uneven", "count": 256, "n_range": (5, 600),
+             "paths_range": (1e5, 1e7), "x_range": (0.01, 0.04), "alpha": 0.99, "seed": 1004}
+# Poisson query stream at a fixed rate, sizes as the 256-portfolio batch, 30 %
+# covariance reuse.
+SERVE_200QPS = {"name": "serve_poisson_200qps", "qps": 200.0, "reuse": 0.3, "n_range": (5, 600),
+                "paths_range": (1e5, 1e7), "x_range": (0.01, 0.04), "alpha": 0.99, "seed": 1005}
 
 
-# Stated latency bound of the serving loop at CONFIG5's rate: p99 of submit ->
+# Stated latency bound of the serving loop at SERVE_200QPS's rate: p99 of submit ->
 # result ready.  The largest queries (n ~ 600, 1e7 paths) need ~15 ms of device
 # time alone, so the bound leaves them room to share the device with the rest.
 P99_BOUND_MS = 50.0
 
 
 def model_of(mcr, cfg):
-    """(mu, sigma) of a single-query config (CONFIG1-3)."""
+    """(mu, sigma) of a single-query workload (PLAIN_N20, FUSED_N50, LARGE_N600)."""
     return mcr.synth_factor_model(cfg["n"], cfg["factors"], cfg["days"], cfg["data_seed"])
 
 
@@ -58,11 +59,11 @@ def _pool():
 
 
 def batch_portfolios(mcr, count=None, seed=None):
-    """Config 4: n log-uniform in [5, 600], paths log-uniform in [1e5, 1e7],
+    """256 uneven portfolios: n log-uniform in [5, 600], paths log-uniform in [1e5, 1e7],
     Dirichlet(1) weights, thresholds U(1 %, 4 %), k = ceil(n / 10); Sigma from
     the factor model with K = clamp(n / 30, 1, 20) factors over T = 2n + 200
     days."""
-    B = CONFIG4
+    B = BATCH_256
     seed = B["seed"] if seed is None else seed
     rng = np.random.default_rng(seed)
     spec = []
@@ -87,10 +88,11 @@ def batch_portfolios(mcr, count=None, seed=None):
 
 
 def serve_queries(mcr, count, seed=None, qps=None):
-    """Config 5 stream: arrival gaps ~ Exp(1 / qps); sizes as config 4; a query
+    """The Poisson query stream: arrival gaps ~ Exp(1 / qps); sizes as the
+    256-portfolio batch; a query
     reuses an earlier query's covariance (and mean) with probability 0.3.
     Returns (queries, gaps in seconds, number of distinct covariances)."""
-    S = CONFIG5
+    S = SERVE_200QPS
     seed = S["seed"] if seed is None else seed
     qps = S["qps"] if qps is None else qps
     rng = np.random.default_rng(seed)

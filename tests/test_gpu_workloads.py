@@ -2,15 +2,15 @@
 queries bench.py measures), each anchored to the C oracle.
 
 Bars:
-  * config 1 (n = 20, 1M paths, k = 5) and config 2 (n = 50, 1e8 paths,
+  * the plain n = 20 workload (1M paths, k = 5) and the fused n = 50 one (1e8 paths,
     k = 10): the fp32 kernel's exceedance and near-threshold counts equal the
     oracle's (serial, and OpenMP over all 1e8 paths) on the identical stream;
-  * config 4 (256 uneven portfolios): every batched result (count, VaR, tail
+  * the 256 uneven portfolios: every batched result (count, VaR, tail
     sum, ES) equals the query run alone; the three smallest-n and three
     largest-n queries are within the bf16x3 tolerances of the FP64 oracle on
     the identical stream (whole query where the oracle finishes in about a
     second, else the first paths of the same query run alone);
-  * config 5 (1000 queries, Poisson arrivals at 200 QPS, 30 % covariance
+  * the query stream (1000 queries, Poisson arrivals at 200 QPS, 30 % covariance
     reuse): every served result equals the query run alone, the same anchors
     against FP64, and the latency p99 within the stated bound.
 bf16x3 tolerances (as in test_gpu_mfma.py): |count - count_fp64| <=
@@ -75,30 +75,30 @@ def extremes(ps, count=3):
     return order[:count] + order[-count:]
 
 
-def test_config1_fp32_equals_serial_oracle(mcr, orc, wl):
-    C = wl.CONFIG1
+def test_plain_n20_fp32_equals_serial_oracle(mcr, orc, wl):
+    C = wl.PLAIN_N20
     mu, sigma = wl.model_of(mcr, C)
     r = mcr.run(mu, sigma, C["x"], C["k"], C["paths"], C["path_seed"], precision=mcr.PREC_FP32,
                 near=True)
     assert r.kernel == "valu_reg"
     ref = orc.simulate_f32(fp32_model(mcr, orc, mu, sigma, C["x"]), C["path_seed"], 0,
                            C["paths"], C["k"], nthreads=1)
-    print(f"config 1: GPU {r.count} (near {r.near_threshold}) vs serial oracle {ref['count']} "
+    print(f"plain n=20: GPU {r.count} (near {r.near_threshold}) vs serial oracle {ref['count']} "
           f"(near {ref['near']})")
     assert (r.count, r.near_threshold) == (ref["count"], ref["near"])
     assert r.p_hat == ref["count"] / C["paths"]
 
 
-def test_config2_full_size_fp32_equals_openmp_oracle(mcr, orc, wl):
+def test_fused_n50_full_size_fp32_equals_openmp_oracle(mcr, orc, wl):
     """All 1e8 paths of the headline stream: the fp32 kernel's counts equal the
     oracle's OpenMP restatement of the serial model (exact integers)."""
-    C = wl.CONFIG2
+    C = wl.FUSED_N50
     mu, sigma = wl.model_of(mcr, C)
     r = mcr.run(mu, sigma, C["x"], C["k"], C["paths"], C["path_seed"], precision=mcr.PREC_FP32,
                 near=True)
     ref = orc.simulate_f32(fp32_model(mcr, orc, mu, sigma, C["x"]), C["path_seed"], 0,
                            C["paths"], C["k"], nthreads=16)
-    print(f"config 2: GPU {r.count} (near {r.near_threshold}) vs oracle {ref['count']} "
+    print(f"fused n=50: GPU {r.count} (near {r.near_threshold}) vs oracle {ref['count']} "
           f"(near {ref['near']}) over {C['paths']} paths")
     assert (r.count, r.near_threshold) == (ref["count"], ref["near"])
     b = mcr.run(mu, sigma, C["x"], C["k"], C["paths"], C["path_seed"], precision=mcr.PREC_BF16X3)
@@ -107,7 +107,7 @@ def test_config2_full_size_fp32_equals_openmp_oracle(mcr, orc, wl):
 
 
 def test_near_flag_on_bf16x3_is_refused_and_auto_uses_fp32(mcr, wl):
-    C = wl.CONFIG2
+    C = wl.FUSED_N50
     mu, sigma = wl.model_of(mcr, C)
     for kw in ({"precision": mcr.PREC_BF16X3}, {"kernel": mcr.KERNEL_MFMA},
                {"kernel": mcr.KERNEL_MFMA_LDS}):
@@ -121,7 +121,7 @@ def test_near_flag_on_bf16x3_is_refused_and_auto_uses_fp32(mcr, wl):
     assert mcr.run(mu, sigma, C["x"], C["k"], 1000, 1, precision=mcr.PREC_AUTO).kernel == "mfma"
 
 
-def test_config4_batch_of_256_equals_alone_and_fp64(mcr, orc, wl):
+def test_batch_of_256_uneven_equals_alone_and_fp64(mcr, orc, wl):
     ps = wl.batch_portfolios(mcr)
     assert len(ps) == 256
     res, o = mcr.run_batch(ps, use_graph=True)
@@ -133,10 +133,10 @@ def test_config4_batch_of_256_equals_alone_and_fp64(mcr, orc, wl):
         same(again[i], r)
     for i in extremes(ps):
         P = anchor_fp64(mcr, orc, ps[i], res[i])
-        print(f"config 4 query {i}: n={len(ps[i]['mu'])} paths={ps[i]['paths']} anchored on {P}")
+        print(f"batch query {i}: n={len(ps[i]['mu'])} paths={ps[i]['paths']} anchored on {P}")
 
 
-def test_config5_stream_of_1000_equals_alone_and_fp64(mcr, orc, wl):
+def test_stream_of_1000_queries_equals_alone_and_fp64(mcr, orc, wl):
     count = 1000
     qs, gaps, _ = wl.serve_queries(mcr, count)
     with mcr.Server() as srv:
@@ -146,7 +146,7 @@ def test_config5_stream_of_1000_equals_alone_and_fp64(mcr, orc, wl):
     assert st.cache_hits > 0
     lat = np.array([x.latency_ms for x in infos])
     p99 = float(np.percentile(lat, 99))
-    print(f"config 5: {count} queries in {wall:.2f} s (offered 200 QPS, submitted at "
+    print(f"query stream: {count} queries in {wall:.2f} s (offered 200 QPS, submitted at "
           f"{count / span:.0f}/s); p50 {np.percentile(lat, 50):.2f} ms, p99 {p99:.1f} ms, "
           f"cache hits {st.cache_hits}")
     for q, r in zip(qs, out):
