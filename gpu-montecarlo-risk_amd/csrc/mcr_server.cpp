@@ -11,7 +11,8 @@
 // so a query's result does not depend on which rounds or which other queries
 // its paths shared the device with.
 //
-// Device work per round r (parity p = r & 1), on three default-priority streams:
+// Device work per round r (parity p = r & 1), on two default-priority streams
+// (plus the class-launch forks):
 //   setup:  H2D of new queries' inputs and descriptors, k_factor_batch
 //           (factorisation skipped on a cache hit), k_prep_batch, H2D of the round's work
 //           items; event setup[p].  Overlaps the previous round's path kernel.
@@ -19,8 +20,8 @@
 //           over the round's items (k_batch_small / k_batch_paths / k_batch_spec),
 //           the classes forked over nfork streams and joined back; events
 //           start[p], end[p].
-//   out:    wait end[p]; D2H of the records of queries whose last paths ran in
-//           round r; event out[p].
+//   out:    on the path stream after end[p]: D2H of the records of queries
+//           whose last paths ran in round r; event out[p].
 // The worker enqueues round r, then waits for round r-1's outputs and
 // finalises it, so the path stream always holds the next round.  Memory,
 // slots and cache references of a query are released only after its last
@@ -60,11 +61,16 @@ constexpr int64_t kDefaultBudget = 4ll << 30;
 constexpr int kDefaultCache = 256;
 constexpr int kDefaultActive = 1024;
 constexpr double kDefaultRoundMs = 4.0;
-// Class-launch streams per round.  One: forking a round over 3 streams made
-// the loop slower (3000-query stream at 800 QPS: p99 33 -> 63 ms; saturation
-// 1090 -> 880 queries/s), since setup, paths, output and the forks then share
-// the process's four hardware queues and setup work waits behind path kernels.
+// Class-launch streams per round: one by default, at most two.  A round uses
+// the setup stream, the path stream (which also carries the output copies)
+// and the forks, and the process has four hardware queues: with three forks
+// and a separate output stream the loop was slower (3000-query stream at 800
+// QPS: p99 33 -> 63 ms; saturation 1090 -> 880 queries/s) as setup work
+// waited behind path kernels.  Moving the output copies onto the path stream
+// took saturation from 1446-1461 to 1510 queries/s (p99 at 1600 QPS offered
+// 441-501 -> 312 ms); two forks then match one (1501 queries/s).
 constexpr int kDefaultForks = 1;
+constexpr int kMaxServerForks = 2;
 constexpr uint32_t kMaxItems = 1u << 16;  // work items per round
 constexpr uint64_t kItemGroups = 32;      // 64-path groups per work item, at most
 constexpr int kMaxRuns = 64;              // kernel classes per round (at most 41 exist)
@@ -338,10 +344,10 @@ int mcr_server::init(int dev, const mcr_server_opts* o) {
   HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   grid = cus;
   grid_override = (o && o->grid > 0) ? o->grid : 0;
-  if (o && o->streams > 0) nfork = std::min(o->streams, kMaxForks);
+  if (o && o->streams > 0) nfork = std::min(o->streams, kMaxServerForks);
   HIP_TRY(hipStreamCreateWithFlags(&s_setup, hipStreamNonBlocking));
   HIP_TRY(hipStreamCreateWithFlags(&s_paths, hipStreamNonBlocking));
-  HIP_TRY(hipStreamCreateWithFlags(&s_out, hipStreamNonBlocking));
+  s_out = s_paths;  // the output copies follow the path kernels
   if (nfork > 1) MCR_TRY(create_forks(&forks, nfork));
   for (int p = 0; p < 2; ++p) {
     HIP_TRY(hipEventCreateWithFlags(&ev_setup[p], hipEventDisableTiming));
@@ -409,7 +415,7 @@ void mcr_server::teardown() {
     if (h_stage[p]) (void)hipHostFree(h_stage[p]);
     if (h_out[p]) (void)hipHostFree(h_out[p]);
   }
-  for (hipStream_t s : {s_setup, s_paths, s_out})
+  for (hipStream_t s : {s_setup, s_paths})
     if (s) (void)hipStreamDestroy(s);
   if (arena) (void)hipFree(arena);
   if (fixed) (void)hipFree(fixed);
