@@ -101,6 +101,20 @@ def test_batch_refinement_fallback(mcr):
     same(res[0], alone(mcr, ps[0]))
 
 
+def test_batch_with_windows_holding_every_loss(mcr):
+    """Windows wide enough to hold every loss (every path appends to the
+    gather buffer), four VaR levels: each batched result equals the query run
+    alone, in one pass."""
+    ps = portfolios(12, sizes=(7, 60, 140, 300, 520), paths=[20000, 30000, 15000, 9000, 6000])
+    for p in ps:
+        p["alphas"] = (0.5, 0.9, 0.99, 0.999)
+        p["extra_flags"] = mcr.F_TEST_WIDE
+    res, _ = mcr.run_batch(ps, use_graph=False, chunk_paths=1024)
+    for p, r in zip(ps, res):
+        assert r.var_passes == 1
+        same(r, alone(mcr, p, extra_flags=mcr.F_TEST_WIDE))
+
+
 def test_per_query_mode_agrees_statistically(mcr):
     ps = portfolios(6, sizes=(30, 90, 250), paths=[200000, 200000, 100000])
     per, o = mcr.run_batch(ps, mode=mcr.BATCH_PER_QUERY)

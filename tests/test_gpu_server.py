@@ -143,6 +143,19 @@ def test_quantile_refinement_fallback(mcr):
         same(r, alone(mcr, p))
 
 
+def test_windows_holding_every_loss(mcr):
+    """Every path's loss appended (windows that hold every loss), three VaR
+    levels: each served result equals the query alone, in one pass."""
+    ps = stream(9, 6, sizes=(20, 130, 450), paths=(5_000, 40_000))
+    for p in ps:
+        p["alphas"] = (0.5, 0.99, 0.999)
+        p["extra_flags"] = mcr.F_TEST_WIDE
+    out, _ = serve(mcr, ps)
+    for p, (r, _) in zip(ps, out):
+        assert r.var_passes == 1
+        same(r, alone(mcr, p))
+
+
 def test_concurrent_submitters_and_poll(mcr):
     ps = stream(8, 24, paths=(1_000, 30_000))
     results = [None] * len(ps)
