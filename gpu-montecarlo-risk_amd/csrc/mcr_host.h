@@ -108,7 +108,11 @@ struct ClassRun {
   int block;            // threads per workgroup of k
   double cost;          // summed cost of the class's items (group_cost units)
 };
-// shapes: per row-tile range, the LDS MFMA kernel shape; nullptr = the tuned ones
+// The tuned LDS MFMA workgroup shape per row-tile range (0 = the default).
+void tuned_shapes(int out[kBigRanges]);
+// shapes: per row-tile range, the LDS MFMA kernel shape; nullptr = the tuned
+// ones (read anew at each call: a caller that needs a kernel and its block
+// size to agree passes one snapshot to both)
 BatchKernel class_kernel(int n, const int* shapes = nullptr);
 int class_block(int n, const int* shapes = nullptr);
 int class_grid(BatchKernel k, int block);

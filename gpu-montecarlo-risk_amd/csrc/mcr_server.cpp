@@ -600,13 +600,18 @@ int mcr_server::plan_round(uint64_t round, RoundRec* rr, std::vector<WorkItem>* 
   // kItemGroups groups.  Sizing from the round's total instead gave a class
   // with little work a few huge items, one wave or workgroup grinding through
   // each while the rest of the device idled inside that launch.
+  // One snapshot of the tuned workgroup shapes for the whole round: the batch
+  // autotuner may change them from another thread, and a class's kernel and
+  // its block size must come from the same shape.
+  int shapes[kBigRanges];
+  tuned_shapes(shapes);
   std::map<BatchKernel, uint64_t> class_groups;
-  for (auto& gr : grant) class_groups[class_kernel(gr.first->q.n)] += gr.second;
+  for (auto& gr : grant) class_groups[class_kernel(gr.first->q.n, shapes)] += gr.second;
   auto item_groups = [&](const Job* j) {
     const int n = j->q.n;
-    const BatchKernel k = class_kernel(n);
+    const BatchKernel k = class_kernel(n, shapes);
     const uint64_t slots =
-        (uint64_t)class_grid(k, class_block(n)) * (n <= kSmallMaxN ? 4 : 1);
+        (uint64_t)class_grid(k, class_block(n, shapes)) * (n <= kSmallMaxN ? 4 : 1);
     return std::max<uint64_t>(1, std::min<uint64_t>(kItemGroups, class_groups[k] / (4 * slots)));
   };
   size_t est = 0;
@@ -642,8 +647,8 @@ int mcr_server::plan_round(uint64_t round, RoundRec* rr, std::vector<WorkItem>* 
   std::vector<int> qb(max_active, 0);
   std::vector<double> qcost(max_active, 0.0);
   for (auto& gr : grant) {
-    qk[gr.first->slot] = class_kernel(gr.first->q.n);
-    qb[gr.first->slot] = class_block(gr.first->q.n);
+    qk[gr.first->slot] = class_kernel(gr.first->q.n, shapes);
+    qb[gr.first->slot] = class_block(gr.first->q.n, shapes);
     qcost[gr.first->slot] = group_cost(gr.first->T, gr.first->S);
   }
   order_by_class(items, qk, qb, qcost, true, &rr->runs);
