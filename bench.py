@@ -145,6 +145,54 @@ def plain_leg(mcr, orc, wl):
     return res
 
 
+def reference_driver_leg(mcr):
+    """The reference's own published setup run through its driver flow: the
+    reference-shaped runner (mcr_runner: main_runner.c's sequence) loads the
+    15-index real-data model (mu, Sigma from the reference's dataset, in
+    tests/golden/realdata_stats.json) as data/*.bin, reads config.yaml (M = 1e8,
+    k = 5, x = 0.02, seed 67) and times run_model of the hip model with
+    CLOCK_MONOTONIC, as main_runner.c does, in a fresh process: context
+    creation, factor, upload, the fp32 path kernel, the result copy; once more
+    with the per-path S values copied back, as the reference's CUDA model
+    always does.  The reference's published CUDA row (N = 9 tickers, same M,
+    k, x, seed, on an unstated NVIDIA GPU) is 2,773 ms."""
+    import csv
+    import subprocess
+    import tempfile
+
+    import numpy as np
+
+    g = json.load(open(os.path.join(ROOT, "tests", "golden", "realdata_stats.json")))
+    c = g["config"]
+    out = {"n_assets": len(g["mu"]), "paths": int(c["M"]), "k": c["k"], "x": c["x"],
+           "seed": c["random_seed"], "reference_cuda_published_ms": 2773.0,
+           "reference_cuda_published_n_assets": 9}
+    for tag, env_extra in (("hip", {}), ("hip_with_S_values", {"MCR_HIP_S_VALUES": "1"})):
+        with tempfile.TemporaryDirectory() as d:
+            mcr.write_bins(os.path.join(d, "data"), np.array(g["mu"]), np.array(g["Sigma"]),
+                           g["actual_freq"])
+            idx = "\n".join(f'  {k}: "{v}"' for k, v in c["indices"].items())
+            with open(os.path.join(d, "config.yaml"), "w") as f:
+                f.write(f"models: [hip]\nstart: \"{c['start']}\"\nend: \"{c['end']}\"\n"
+                        f"indices:\n{idx}\ntrain_ratio: {c['train_ratio']}\nM: {c['M']}\n"
+                        f"random_seed: {c['random_seed']}\nx: {c['x']}\nk: {c['k']}\n")
+            env = dict(os.environ, **env_extra)
+            r = subprocess.run([mcr.RUNNER_PATH], cwd=d, input="bench\n", capture_output=True,
+                               text=True, timeout=300, env=env)
+            if r.returncode != 0:
+                raise SystemExit(f"reference-driver leg failed: {r.stderr[-500:]}")
+            with open(os.path.join(d, "results", "simulation_results.csv"), newline="") as f:
+                row = list(csv.DictReader(f))[0]
+        ms = float(row["execution_time_ms"])
+        out[tag] = {"run_model_ms": ms, "paths_per_s": c["M"] / (ms * 1e-3),
+                    "count": int(row["count"]), "P_hat": float(row["P_hat"]),
+                    "vs_reference_cuda_published": 2773.0 / ms}
+    print(f"# reference driver (N={out['n_assets']}, M=1e8, fresh process): run_model "
+          f"{out['hip']['run_model_ms']:.0f} ms, with S values {out['hip_with_S_values']['run_model_ms']:.0f} ms "
+          f"(reference CUDA, N=9: 2773 ms)", file=sys.stderr, flush=True)
+    return out
+
+
 def same_result(a, b):
     return (a.count == b.count and a.var == b.var and a.es == b.es
             and a.tail_sum_fixed == b.tail_sum_fixed)
@@ -409,6 +457,7 @@ def main():
     orc = None if (args.no_cpu and args.no_plain) else ge.load_oracle()
     if not args.no_plain:
         out["plain"] = plain_leg(mcr, orc, wl)
+        out["reference_driver"] = reference_driver_leg(mcr)
     if not args.no_large:
         out["large"] = large_leg(mcr, np, wl)
     if not args.no_batch:
