@@ -270,13 +270,21 @@ def serve_leg(mcr, np, wl, count, rates):
               "sustained_paths_per_s": paths / wall,
               "device_busy_frac": (st.device_ms - st0.device_ms) / (wall * 1e3),
               "rounds": st.rounds - st0.rounds,
+              # worker-thread time per round, by phase (admit, plan, enqueue,
+              # wait for the previous round's results, finalise): the loop is
+              # host-paced once all but the wait add up to a round's device time
+              "host_ms_per_round": {ph: (getattr(st, f"host_{ph}_ms") - getattr(st0, f"host_{ph}_ms"))
+                                    / max(1, st.rounds - st0.rounds)
+                                    for ph in ("admit", "plan", "enqueue", "wait", "finalize")},
               "cache_hit_rate": (st.cache_hits - st0.cache_hits) / count,
               "peak_device_bytes": st.peak_bytes, "failed": st.failed - st0.failed}
         points.append(pt)
         print(f"# serve {rate:.0f} QPS offered: {pt['achieved_qps']:.0f} submitted/s, "
               f"{pt['completed_qps']:.0f} done/s, p50 {pt['latency_ms']['p50']:.2f} ms, p99 "
               f"{pt['latency_ms']['p99']:.1f} ms, {pt['sustained_paths_per_s']:.3g} paths/s, busy "
-              f"{pt['device_busy_frac']:.2f}", file=sys.stderr, flush=True)
+              f"{pt['device_busy_frac']:.2f}, {pt['rounds']} rounds, host "
+              + "/".join(f"{v:.2f}" for v in pt["host_ms_per_round"].values())
+              + " ms per round (admit/plan/enqueue/wait/finalize)", file=sys.stderr, flush=True)
     head = points[0]
     # saturation: the highest offered rate the loop still completes at >= 95 %
     kept = [p for p in points if p["completed_qps"] >= 0.95 * p["offered_qps"]]

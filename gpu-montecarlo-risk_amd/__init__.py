@@ -386,7 +386,10 @@ class ServerStats(ctypes.Structure):
                 ("cache_hits", ctypes.c_int64), ("cache_misses", ctypes.c_int64),
                 ("paths_done", ctypes.c_int64), ("device_ms", ctypes.c_double),
                 ("peak_bytes", ctypes.c_int64), ("budget_bytes", ctypes.c_int64),
-                ("active", ctypes.c_int), ("queued", ctypes.c_int)]
+                ("active", ctypes.c_int), ("queued", ctypes.c_int),
+                ("host_admit_ms", ctypes.c_double), ("host_plan_ms", ctypes.c_double),
+                ("host_enqueue_ms", ctypes.c_double), ("host_wait_ms", ctypes.c_double),
+                ("host_finalize_ms", ctypes.c_double)]
 
 
 def _server_lib():

@@ -287,6 +287,7 @@ struct mcr_server {
   std::vector<Job*> active;
   double ms_per_unit = kInitMsPerUnit;
   uint64_t use_clock = 0;
+  Clock::time_point t_ready{};  // end of finalize_round's wait for the results
 
   // shared state (mu)
   std::mutex mu;
@@ -817,6 +818,7 @@ void mcr_server::finish_job(Job* j, int status, const char* err) {
 int mcr_server::finalize_round(RoundRec& rr) {
   const int p = rr.parity;
   HIP_TRY(hipEventSynchronize(ev_out[p]));
+  t_ready = Clock::now();
   float ms = 0.0f;
   HIP_TRY(hipEventElapsedTime(&ms, ev_start[p], ev_end[p]));
   {
@@ -921,14 +923,30 @@ void mcr_server::run() {
       if (stop && active.empty() && !inflight) return;
     }
     std::vector<Job*> fresh;
+    const Clock::time_point t0 = Clock::now();
     int rc = admit(round, &fresh);
+    const Clock::time_point t1 = Clock::now();
     auto rr = std::make_unique<RoundRec>();
     if (rc == MCR_OK) rc = plan_round(round, rr.get(), &items);
+    const Clock::time_point t2 = Clock::now();
     const bool have = !items.empty() || !fresh.empty();
     if (rc == MCR_OK && have) rc = enqueue_round(*rr, fresh, items);
+    const Clock::time_point t3 = Clock::now();
     if (rc == MCR_OK && inflight) {
       rc = finalize_round(*inflight);
       inflight.reset();
+    }
+    const Clock::time_point t4 = Clock::now();
+    {
+      // t_ready: when finalize_round's wait for the results ended (t3 if
+      // there was no round to finalise)
+      const Clock::time_point tw = t_ready > t3 ? t_ready : t3;
+      std::lock_guard<std::mutex> lk(mu);
+      stats.host_admit_ms += ms_since(t0, t1);
+      stats.host_plan_ms += ms_since(t1, t2);
+      stats.host_enqueue_ms += ms_since(t2, t3);
+      stats.host_wait_ms += ms_since(t3, tw);
+      stats.host_finalize_ms += ms_since(tw, t4);
     }
     if (rc != MCR_OK) {
       fail_everything(g_err);

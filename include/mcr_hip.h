@@ -247,6 +247,12 @@ typedef struct mcr_server_stats {
   int64_t peak_bytes;    /* high-water mark of the budget in use */
   int64_t budget_bytes;
   int active, queued;    /* at the time of the call */
+  /* Worker-thread time summed over rounds (ms): admission with the
+   * factor-cache lookups, round planning, staging and launching, waiting for
+   * the previous round's results, finalising it (quantiles, releases).  A
+   * round's host work overlaps the previous round's kernels; once it is
+   * longer, the device waits. */
+  double host_admit_ms, host_plan_ms, host_enqueue_ms, host_wait_ms, host_finalize_ms;
 } mcr_server_stats;
 
 /* Device bytes a query needs while admitted (inputs, operands, accumulators,

@@ -121,3 +121,38 @@ def test_noalloc_entry_point_needs_init(mcr):
     assert code == mcr.MCR_E_ARG
     assert b"mcr_init" in mcr.lib().mcr_last_error()
     del keep
+
+
+def test_ctypes_structs_match_the_c_header(mcr, tmp_path):
+    """Every struct of include/mcr_hip.h that the Python view mirrors has the
+    same size and field offsets as gcc lays it out (a C probe compiled here)."""
+    import shutil
+    import subprocess
+
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    pairs = {"mcr_query": mcr.Query, "mcr_result": mcr.Result, "mcr_factor_info": mcr.FactorInfo,
+             "mcr_model_export": mcr.ModelExport, "mcr_batch_opts": mcr.BatchOpts,
+             "mcr_server_opts": mcr.ServerOpts, "mcr_ticket_info": mcr.TicketInfo,
+             "mcr_server_stats": mcr.ServerStats}
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "mcr_hip.h"', "int main(void) {"]
+    for cname, py in pairs.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'  printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "probe.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "probe"
+    subprocess.run(["gcc", "-std=c11", "-I", inc, str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {}
+    for line in out:
+        if line:
+            cname, what, val = line.split()
+            got[(cname, what)] = int(val)
+    for cname, py in pairs.items():
+        assert ctypes.sizeof(py) == got[(cname, "size")], cname
+        for fname, _ in py._fields_:
+            assert getattr(py, fname).offset == got[(cname, fname)], (cname, fname)
