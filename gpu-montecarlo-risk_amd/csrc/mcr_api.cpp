@@ -508,25 +508,23 @@ int run_paths(const mcr_query* q, const Staged& st, PathArgs a, bool per_path, b
 
 int64_t fixed32_host(float v) { return (int64_t)((double)v * 4294967296.0); }
 
-// Resolves a quantile from a bracket pass (window counts in acc slot `slot`,
-// in-window losses in gather[0 .. min(G, cap))); returns 1 when resolved, 0
-// when the rank missed the window or the buffer overflowed, -1 on copy error.
-int resolve_quantile(int64_t M, const DevAccum& acc, int slot, const float* gather, unsigned cap,
-                     int64_t R, double* var, double* es, int64_t* tail, float* hostbuf) {
+// Whether a bracket pass's window (counts in acc slot `slot`) holds rank R
+// with every in-window loss gathered (at most cap of them).
+bool window_holds(int64_t M, const DevAccum& acc, int slot, unsigned cap, int64_t R) {
   const int64_t G = (int64_t)acc.gathered[slot];
-  const int64_t above = (int64_t)acc.above[slot];
-  const int64_t below = M - above - G;
-  if (G > (int64_t)cap || !(below < R && R <= below + G)) return 0;
-  // hostbuf (>= cap floats) serves the allocation-free mode; otherwise a heap copy
-  std::vector<float> heap(hostbuf ? 0 : (size_t)G);
-  float* g = hostbuf ? hostbuf : heap.data();
-  if (G > 0) {
-    if (hipMemcpy(g, gather, (size_t)G * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
-      return -1;
-  }
-  // The (idx+1)-th smallest in-window loss is the quantile; the values after
-  // it in the partition are exactly the larger ones, and their fixed-point
-  // sum does not depend on order, so a partition gives what a sort would.
+  const int64_t below = M - (int64_t)acc.above[slot] - G;
+  return G <= (int64_t)cap && below < R && R <= below + G;
+}
+
+// The quantile, tail sum and ES from a window that holds rank R, its G
+// gathered losses in host memory g (reordered in place).  The (idx+1)-th
+// smallest in-window loss is the quantile; the values after it in the
+// partition are exactly the larger ones, and their fixed-point sum does not
+// depend on order, so a partition gives what a sort would.
+void resolve_window(int64_t M, const DevAccum& acc, int slot, float* g, int64_t R, double* var,
+                    double* es, int64_t* tail) {
+  const int64_t G = (int64_t)acc.gathered[slot];
+  const int64_t below = M - (int64_t)acc.above[slot] - G;
   const int64_t idx = R - below - 1;
   std::nth_element(g, g + idx, g + G);
   int64_t t = acc.sum_above[slot];
@@ -534,7 +532,42 @@ int resolve_quantile(int64_t M, const DevAccum& acc, int slot, const float* gath
   *var = (double)g[(size_t)idx];
   *tail = t;
   *es = (double)t * std::ldexp(1.0, -32) / (double)(M - R + 1);
+}
+
+// Resolves a quantile from a bracket pass (window counts in acc slot `slot`,
+// in-window losses in gather[0 .. min(G, cap))); returns 1 when resolved, 0
+// when the rank missed the window or the buffer overflowed, -1 on copy error.
+int resolve_quantile(int64_t M, const DevAccum& acc, int slot, const float* gather, unsigned cap,
+                     int64_t R, double* var, double* es, int64_t* tail, float* hostbuf) {
+  if (!window_holds(M, acc, slot, cap, R)) return 0;
+  const int64_t G = (int64_t)acc.gathered[slot];  // >= 1: the window holds rank R
+  // hostbuf (>= cap floats) serves the allocation-free mode; otherwise a heap copy
+  std::vector<float> heap(hostbuf ? 0 : (size_t)G);
+  float* g = hostbuf ? hostbuf : heap.data();
+  if (hipMemcpy(g, gather, (size_t)G * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+    return -1;
+  resolve_window(M, acc, slot, g, R, var, es, tail);
   return 1;
+}
+
+int fetch_windows(const std::vector<WindowFetch>& w, float** hbuf, size_t* hcap,
+                  hipStream_t stream) {
+  size_t need = 0;
+  for (const WindowFetch& f : w) need = std::max(need, f.off + (size_t)f.count);
+  if (need == 0) return MCR_OK;
+  if (need > *hcap) {
+    if (*hbuf) HIP_TRY(hipHostFree(*hbuf));
+    *hbuf = nullptr;
+    *hcap = 0;
+    const size_t cap = std::max<size_t>(need + need / 2, 1 << 16);
+    HIP_TRY(hipHostMalloc((void**)hbuf, cap * sizeof(float), hipHostMallocDefault));
+    *hcap = cap;
+  }
+  for (const WindowFetch& f : w)
+    HIP_TRY(hipMemcpyAsync(*hbuf + f.off, f.dev, (size_t)f.count * sizeof(float),
+                           hipMemcpyDeviceToHost, stream));
+  HIP_TRY(hipStreamSynchronize(stream));
+  return MCR_OK;
 }
 
 int try_resolve(const mcr_query* q, const Staged& st, const DevAccum& acc, int slot, int64_t R,
@@ -732,6 +765,26 @@ void set_windows(const mcr_query* q, int n_alpha, FactorArgs* f) {
                           ? 1e6
                           : 6.0 * std::sqrt(av * (1.0 - av) / (double)q->paths) / dens + 1e-4;
   }
+}
+
+// Gather capacity per VaR level for a query whose results are copied back
+// with its record: the expected number of in-window losses (the portfolio
+// loss is Gaussian, the window is za +- ha standard deviations) plus eight
+// standard deviations and 64, at most cap.  A window that still overflows is
+// resolved by the refinement passes, as any missed window.
+unsigned window_capacity(const mcr_query* q, unsigned cap) {
+  const int na = (q->flags & MCR_F_VAR) ? q->n_alpha : 0;
+  if (na <= 0) return 0;
+  FactorArgs f = {};
+  set_windows(q, na, &f);
+  double need = 0.0;
+  for (int al = 0; al < na; ++al) {
+    const double frac = 0.5 * (std::erfc(-(f.za[al] + f.ha[al]) / std::sqrt(2.0)) -
+                               std::erfc(-(f.za[al] - f.ha[al]) / std::sqrt(2.0)));
+    const double eg = (double)q->paths * std::max(0.0, frac);
+    need = std::max(need, eg + 8.0 * std::sqrt(eg) + 64.0);
+  }
+  return (unsigned)std::min<double>((double)cap, std::ceil(need));
 }
 
 // P-hat, its standard error and 95% interval (src/05-GPU/monte_carlos_cuda.cu:146-150),
@@ -1237,6 +1290,15 @@ int run_batch_locked(const mcr_query* qs, int nq, mcr_result* rs, mcr_batch_opts
   const DevInfo* infos = (const DevInfo*)hout;
   const DevAccum* accs = (const DevAccum*)(hout + (bp.offAcc - bp.offInfo));
   std::vector<int> refine;
+  // every window that holds its rank is copied to host memory together (one
+  // wait), then resolved; a query with a missed window is recomputed below
+  struct Pending {
+    int query, level;
+    int64_t R;
+  };
+  std::vector<WindowFetch> fetch;
+  std::vector<Pending> pend;
+  size_t off = 0;
   for (int i = 0; i < nq; ++i) {
     const mcr_query* q = &qs[i];
     mcr_result* r = &rs[i];
@@ -1254,18 +1316,25 @@ int run_batch_locked(const mcr_query* qs, int nq, mcr_result* rs, mcr_batch_opts
     r->ms_kernel = ms;
     r->var_passes = bp.na[i] ? 1 : 0;
     bool ok = true;
-    for (int al = 0; al < bp.na[i] && ok; ++al) {
-      const int64_t R = rank_of(q->alpha[al], M);
-      const float* gd = (const float*)(dev + bp.offGather[i]) + (size_t)al * kBatchGatherCap;
-      const int res = resolve_quantile(M, accs[i], al, gd, kBatchGatherCap, R, &r->var[al],
-                                       &r->es[al], &r->tail_sum_fixed[al]);
-      if (res < 0) {
-        set_error("copy of gathered losses failed");
-        return MCR_E_DEVICE;
-      }
-      if (res == 0) ok = false;
+    for (int al = 0; al < bp.na[i]; ++al)
+      ok = ok && window_holds(M, accs[i], al, kBatchGatherCap, rank_of(q->alpha[al], M));
+    if (!ok) {
+      refine.push_back(i);
+      continue;
     }
-    if (!ok) refine.push_back(i);
+    for (int al = 0; al < bp.na[i]; ++al) {
+      const int64_t G = (int64_t)accs[i].gathered[al];
+      fetch.push_back({(const float*)(dev + bp.offGather[i]) + (size_t)al * kBatchGatherCap, G, off});
+      pend.push_back({i, al, rank_of(q->alpha[al], M)});
+      off += (size_t)G;
+    }
+  }
+  MCR_TRY(fetch_windows(fetch, &g_ctx.hwin, &g_ctx.hwin_cap, g_ctx.stream));
+  for (size_t k = 0; k < pend.size(); ++k) {
+    const Pending& pw = pend[k];
+    mcr_result* r = &rs[pw.query];
+    resolve_window(qs[pw.query].paths, accs[pw.query], pw.level, g_ctx.hwin + fetch[k].off, pw.R,
+                   &r->var[pw.level], &r->es[pw.level], &r->tail_sum_fixed[pw.level]);
   }
   // Rare: a quantile missed its window.  Recompute that query alone on the
   // same kernel (identical per-path losses) with the refinement passes.
@@ -1315,6 +1384,7 @@ int mcr_shutdown(void) {
   drop_graph();
   if (g_ctx.arena) HIP_TRY(hipFree(g_ctx.arena));
   if (g_ctx.host) HIP_TRY(hipHostFree(g_ctx.host));
+  if (g_ctx.hwin) HIP_TRY(hipHostFree(g_ctx.hwin));
   HIP_TRY(hipEventDestroy(g_ctx.ev0));
   HIP_TRY(hipEventDestroy(g_ctx.ev1));
   HIP_TRY(hipEventDestroy(g_ctx.evs));

@@ -69,6 +69,8 @@ struct Context {
   size_t arena_bytes = 0;
   void* host = nullptr;  // pinned staging
   size_t host_bytes = 0;
+  float* hwin = nullptr;  // pinned copies of batches' gathered in-window losses
+  size_t hwin_cap = 0;    // floats
   uint64_t arena_gen = 0;  // bumped whenever the arena moves (invalidates graphs)
   ForkSet forks;           // class-launch streams of batches
   // Device address of the file-scope scratch array of mcr_api.cpp on this
@@ -91,6 +93,22 @@ double group_cost(int T, int S);
 // hostbuf: nullptr, or a host buffer of >= cap floats used instead of a heap copy
 int resolve_quantile(int64_t M, const DevAccum& acc, int slot, const float* gather, unsigned cap,
                      int64_t R, double* var, double* es, int64_t* tail, float* hostbuf = nullptr);
+bool window_holds(int64_t M, const DevAccum& acc, int slot, unsigned cap, int64_t R);
+// gather capacity per VaR level that holds a query's window losses (<= cap)
+unsigned window_capacity(const mcr_query* q, unsigned cap);
+void resolve_window(int64_t M, const DevAccum& acc, int slot, float* g, int64_t R, double* var,
+                    double* es, int64_t* tail);
+// One device window of gathered losses to copy: count floats from dev to
+// offset off of a pinned host buffer.
+struct WindowFetch {
+  const float* dev;
+  int64_t count;
+  size_t off;
+};
+// Copies every window on `stream` (asynchronously, one wait at the end) into
+// *hbuf, a pinned buffer of *hcap floats grown as needed.
+int fetch_windows(const std::vector<WindowFetch>& w, float** hbuf, size_t* hcap,
+                  hipStream_t stream);
 // noalloc: device buffers from the scratch array, host buffers static (no
 // hipMalloc, pinned or heap allocation); the library must be initialised
 int run_locked(const mcr_query* q, mcr_result* r, bool noalloc = false);

@@ -131,14 +131,16 @@ size_t entry_bytes(int n) {
   return align_up(sizeof(DevInfo)) + align_up((size_t)n * n * sizeof(double));
 }
 
-// Region of an admitted query.  Inputs first (one H2D span), then the rest.
+// Region of an admitted query.  Inputs first (one H2D span), then its record,
+// accumulators and gathered window losses (one D2H span), then the rest.
 struct JobLayout {
   size_t mu = 0, w = 0, A = 0, in_span = 0;
-  size_t info = 0, acc = 0, out_span = 0;
-  size_t v64 = 0, rowinit = 0, lfrag = 0, gather = 0, total = 0;
+  size_t info = 0, acc = 0, gather = 0, out_span = 0;
+  size_t v64 = 0, rowinit = 0, lfrag = 0, total = 0;
 };
 
-JobLayout job_layout(int n, int T, int S, int na, bool with_A) {
+// wcap: gathered losses per VaR level (window_capacity)
+JobLayout job_layout(int n, int T, int S, int na, bool with_A, unsigned wcap) {
   JobLayout l;
   Carver c{nullptr};
   l.mu = c.off;
@@ -152,6 +154,8 @@ JobLayout job_layout(int n, int T, int S, int na, bool with_A) {
   c.take<DevInfo>(1);
   l.acc = c.off;
   c.take<DevAccum>(1);
+  l.gather = c.off;
+  if (na > 0) c.take<float>((size_t)na * wcap);
   l.out_span = c.off - l.info;
   l.v64 = c.off;
   c.take<double>(n);
@@ -159,8 +163,6 @@ JobLayout job_layout(int n, int T, int S, int na, bool with_A) {
   c.take<float>((size_t)32 * T);
   l.lfrag = c.off;
   c.take<uint4>((size_t)T * S * 128);
-  l.gather = c.off;
-  if (na > 0) c.take<float>((size_t)na * kBatchGatherCap);
   l.total = c.off;
   return l;
 }
@@ -187,6 +189,7 @@ struct Job {
   std::vector<double> mu, w, sigma;  // compact copies; q points at them
   uint64_t hash = 0;
   int T = 0, S = 0, na = 0;
+  unsigned wcap = 0;  // gathered losses per VaR level (window_capacity)
   JobState state = JobState::kQueued;
   int status = MCR_OK;
   std::string err;
@@ -211,6 +214,7 @@ struct RoundRec {
   double units = 0.0;
   size_t nitems = 0;
   std::vector<ClassRun> runs;
+  std::vector<size_t> out_off;  // per finishing query: its output span in h_out
 };
 
 uint64_t hash_sigma(int n, const double* s) {
@@ -247,7 +251,8 @@ int check_servable(const mcr_query* q) {
 
 size_t query_footprint(const mcr_query* q, bool with_factor) {
   const int na = (q->flags & MCR_F_VAR) ? q->n_alpha : 0;
-  const JobLayout l = job_layout(q->n, tiles_T(q->n), steps_S(q->n), na, with_factor);
+  const JobLayout l = job_layout(q->n, tiles_T(q->n), steps_S(q->n), na, with_factor,
+                                 window_capacity(q, kBatchGatherCap));
   return l.total + (with_factor ? entry_bytes(q->n) : 0);
 }
 
@@ -468,7 +473,7 @@ bool mcr_server::try_admit(Job* j, uint64_t round) {
   }
   if (ce && ce->ready_round > round) return false;  // its factor is being made this round
   const bool hit = ce != nullptr;
-  const JobLayout lay = job_layout(j->q.n, j->T, j->S, j->na, !hit);
+  const JobLayout lay = job_layout(j->q.n, j->T, j->S, j->na, !hit, j->wcap);
   for (;;) {
     size_t joff = 0, eoff = 0;
     bool ok = alloc.alloc(lay.total, &joff);
@@ -742,7 +747,7 @@ int mcr_server::enqueue_round(RoundRec& rr, const std::vector<Job*>& fresh,
     a.loss_mode = j->na ? kLossBracket : kLossNone;
     a.n_alpha = j->na;
     a.gather = j->na ? (float*)(dj + j->lay.gather) : nullptr;
-    a.gather_cap = kBatchGatherCap;
+    a.gather_cap = j->wcap;
     a.hist = d_hist;
     qa[i] = a;
     max_n = std::max(max_n, q->n);
@@ -772,12 +777,21 @@ int mcr_server::enqueue_round(RoundRec& rr, const std::vector<Job*>& fresh,
                       grid_override, s_paths, &forks, nfork));
   HIP_TRY(hipEventRecord(ev_end[p], s_paths));
 
-  const size_t span = align_up(sizeof(DevInfo)) + sizeof(DevAccum);
-  MCR_TRY(grow_pinned(&h_out[p], &h_out_bytes[p], rr.finishing.size() * align_up(span)));
+  // each finishing query's record, accumulators and gathered window losses
+  // in one copy, in order behind the path kernels: finalising the round then
+  // needs no copy of its own (a synchronous copy on another stream can land
+  // on the hardware queue of the path stream and wait for the next round)
+  rr.out_off.resize(rr.finishing.size());
+  size_t out_bytes = 0;
+  for (size_t i = 0; i < rr.finishing.size(); ++i) {
+    rr.out_off[i] = out_bytes;
+    out_bytes += align_up(rr.finishing[i]->lay.out_span);
+  }
+  MCR_TRY(grow_pinned(&h_out[p], &h_out_bytes[p], out_bytes));
   HIP_TRY(hipStreamWaitEvent(s_out, ev_end[p], 0));
   for (size_t i = 0; i < rr.finishing.size(); ++i) {
     Job* j = rr.finishing[i];
-    HIP_TRY(hipMemcpyAsync(h_out[p] + i * align_up(span), arena + j->off + j->lay.info, span,
+    HIP_TRY(hipMemcpyAsync(h_out[p] + rr.out_off[i], arena + j->off + j->lay.info, j->lay.out_span,
                            hipMemcpyDeviceToHost, s_out));
   }
   HIP_TRY(hipEventRecord(ev_out[p], s_out));
@@ -833,11 +847,12 @@ int mcr_server::finalize_round(RoundRec& rr) {
     const double obs = (double)ms * grid / rr.units;
     ms_per_unit = 0.8 * ms_per_unit + 0.2 * obs;
   }
-  const size_t span = align_up(align_up(sizeof(DevInfo)) + sizeof(DevAccum));
   for (size_t i = 0; i < rr.finishing.size(); ++i) {
     Job* j = rr.finishing[i];
-    const DevInfo& info = *(const DevInfo*)(h_out[p] + i * span);
-    const DevAccum& acc = *(const DevAccum*)(h_out[p] + i * span + align_up(sizeof(DevInfo)));
+    const char* o = h_out[p] + rr.out_off[i];
+    const DevInfo& info = *(const DevInfo*)o;
+    const DevAccum& acc = *(const DevAccum*)(o + (j->lay.acc - j->lay.info));
+    float* win = (float*)(h_out[p] + rr.out_off[i] + (j->lay.gather - j->lay.info));
     mcr_result* r = &j->res;
     std::memset(r, 0, sizeof(*r));
     const int fr = check_factor(info);
@@ -850,21 +865,16 @@ int mcr_server::finalize_round(RoundRec& rr) {
     r->kernel = j->q.n <= kSmallMaxN ? MCR_KERNEL_MFMA : MCR_KERNEL_MFMA_LDS;
     r->var_passes = j->na ? 1 : 0;
     bool ok = true;
-    for (int al = 0; al < j->na && ok; ++al) {
-      const int64_t R = rank_of(j->q.alpha[al], M);
-      const float* gd = (const float*)(arena + j->off + j->lay.gather) + (size_t)al * kBatchGatherCap;
-      const int res = resolve_quantile(M, acc, al, gd, kBatchGatherCap, R, &r->var[al], &r->es[al],
-                                       &r->tail_sum_fixed[al]);
-      if (res < 0) {
-        set_error("copy of gathered losses failed");
-        return MCR_E_DEVICE;
-      }
-      if (res == 0) ok = false;
-    }
+    for (int al = 0; al < j->na; ++al)
+      ok = ok && window_holds(M, acc, al, j->wcap, rank_of(j->q.alpha[al], M));
     int status = MCR_OK;
-    if (!ok) {
-      // Rare: a quantile missed its window.  Same kernel family alone, with
-      // the refinement passes (identical per-path losses).
+    if (ok) {
+      for (int al = 0; al < j->na; ++al)
+        resolve_window(M, acc, al, win + (size_t)al * j->wcap, rank_of(j->q.alpha[al], M),
+                       &r->var[al], &r->es[al], &r->tail_sum_fixed[al]);
+    } else {
+      // Rare: a quantile missed (or overflowed) its window.  Same kernel
+      // family alone, with the refinement passes (identical per-path losses).
       mcr_query q1 = j->q;
       q1.kernel = MCR_KERNEL_AUTO;
       q1.precision = MCR_PREC_BF16X3;
@@ -1022,6 +1032,7 @@ int mcr_server_submit(mcr_server* s, const mcr_query* q, uint64_t* ticket) {
   j->T = tiles_T(n);
   j->S = steps_S(n);
   j->na = (q->flags & MCR_F_VAR) ? q->n_alpha : 0;
+  j->wcap = window_capacity(&j->q, kBatchGatherCap);
   j->t_submit = Clock::now();
   {
     std::lock_guard<std::mutex> lk(s->mu);

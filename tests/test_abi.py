@@ -93,10 +93,22 @@ def test_server_footprint_and_servable_checks(mcr):
     p = {"mu": np.zeros(n), "sigma": np.eye(n), "x": 0.02, "k": 5, "paths": 10**7, "seed": 1,
          "alphas": (0.99,), "precision": mcr.PREC_BF16X3}
     b = mcr.server_footprint(p)
-    # factor (FP64 L) + covariance copy + bf16 hi/lo operands + one VaR buffer dominate
+    # factor (FP64 L) + covariance copy + bf16 hi/lo operands + one VaR buffer
+    # dominate; the VaR buffer holds the window's expected count (the loss is
+    # Gaussian: window za +- ha standard deviations) plus 8 sigma and 64
+    from scipy.stats import norm
     T, S = (n + 1 + 31) // 32, (n + 15) // 16
-    assert b >= 2 * n * n * 8 + T * S * 128 * 16 + 65536 * 4
-    assert b <= 1.1 * (2 * n * n * 8 + T * S * 128 * 16 + 65536 * 4) + 8192
+    za = norm.ppf(0.99)
+    ha = 6.0 * np.sqrt(0.99 * 0.01 / 1e7) / norm.pdf(za) + 1e-4
+    eg = 1e7 * (norm.cdf(za + ha) - norm.cdf(za - ha))
+    wcap = int(np.ceil(eg + 8 * np.sqrt(eg) + 64))
+    assert 3000 < eg < 5000 and wcap < 65536
+    base = 2 * n * n * 8 + T * S * 128 * 16
+    assert b >= base + wcap * 4
+    assert b <= 1.1 * (base + wcap * 4) + 8192
+    # windows that hold every loss get the whole 2^16-loss buffer
+    wide = mcr.server_footprint(dict(p, extra_flags=mcr.F_TEST_WIDE))
+    assert wide >= base + 65536 * 4 and wide - b >= (65536 - wcap) * 4 - 256
     small = dict(p, mu=np.zeros(5), sigma=np.eye(5), alphas=())
     assert mcr.server_footprint(small) < 16384
     for bad, code in (({"precision": mcr.PREC_FP32}, mcr.MCR_E_UNSUPPORTED),
