@@ -61,15 +61,16 @@ constexpr int64_t kDefaultBudget = 4ll << 30;
 constexpr int kDefaultCache = 256;
 constexpr int kDefaultActive = 1024;
 constexpr double kDefaultRoundMs = 4.0;
-// Class-launch streams per round: one by default, at most two.  A round uses
+// Class-launch streams per round: two by default, at most two.  A round uses
 // the setup stream, the path stream (which also carries the output copies)
 // and the forks, and the process has four hardware queues: with three forks
 // and a separate output stream the loop was slower (3000-query stream at 800
 // QPS: p99 33 -> 63 ms; saturation 1090 -> 880 queries/s) as setup work
-// waited behind path kernels.  Moving the output copies onto the path stream
-// took saturation from 1446-1461 to 1510 queries/s (p99 at 1600 QPS offered
-// 441-501 -> 312 ms); two forks then match one (1501 queries/s).
-constexpr int kDefaultForks = 1;
+// waited behind path kernels.  With the output copies on the path stream and
+// finalising free of copies, two forks against one: 1539-1543 vs 1530-1531
+// queries/s completed at 1600 QPS offered, p99 116-118 vs 180-231 ms; at 800
+// QPS p99 24-26 ms either way.
+constexpr int kDefaultForks = 2;
 constexpr int kMaxServerForks = 2;
 constexpr uint32_t kMaxItems = 1u << 16;  // work items per round
 constexpr uint64_t kItemGroups = 32;      // 64-path groups per work item, at most

@@ -228,7 +228,7 @@ typedef struct mcr_server_opts {
   double round_ms;     /* target device time of one round; 0 = 4 ms */
   int grid;            /* persistent workgroups; 0 = one per CU */
   int streams;         /* concurrent streams a round's class launches are forked
-                          over (1..2; 0 = 1, larger values are taken as 2) */
+                          over (1..2; 0 = 2, larger values are taken as 2) */
 } mcr_server_opts;
 
 typedef struct mcr_ticket_info {
