@@ -115,6 +115,21 @@ def test_batch_with_windows_holding_every_loss(mcr):
         same(r, alone(mcr, p, extra_flags=mcr.F_TEST_WIDE))
 
 
+def test_batch_window_overflow_falls_back_to_refinement(mcr):
+    """Every loss of 100k paths in the window overflows a batched query's
+    2^16-loss buffer: the query is recomputed alone with the refinement
+    passes, its results equal the query alone, and the other queries of the
+    batch resolve in one pass."""
+    ps = portfolios(4, sizes=(30, 200), paths=[100000, 20000])
+    for p in ps:
+        p["alphas"] = (0.99,)
+        p["extra_flags"] = mcr.F_TEST_WIDE
+    res, _ = mcr.run_batch(ps, use_graph=False, chunk_paths=1024)
+    for p, r in zip(ps, res):
+        assert r.var_passes == (1 if p["paths"] <= 65536 else 2)
+        same(r, alone(mcr, p, extra_flags=mcr.F_TEST_WIDE))
+
+
 def test_per_query_mode_agrees_statistically(mcr):
     ps = portfolios(6, sizes=(30, 90, 250), paths=[200000, 200000, 100000])
     per, o = mcr.run_batch(ps, mode=mcr.BATCH_PER_QUERY)

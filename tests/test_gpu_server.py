@@ -156,6 +156,22 @@ def test_windows_holding_every_loss(mcr):
         same(r, alone(mcr, p))
 
 
+def test_window_overflow_falls_back_to_refinement(mcr):
+    """Windows that hold every loss of 100k-150k paths overflow the served
+    query's 2^16-loss window buffer (a single query keeps 2^18): the server
+    sends those queries through the refinement passes, and every result still
+    equals the query alone (which resolves in one pass)."""
+    ps = stream(10, 4, sizes=(20, 140), paths=(100_000, 150_000))
+    for p in ps:
+        p["alphas"] = (0.5, 0.99)
+        p["extra_flags"] = mcr.F_TEST_WIDE
+    out, _ = serve(mcr, ps)
+    for p, (r, _) in zip(ps, out):
+        a = alone(mcr, p)
+        assert a.var_passes == 1 and r.var_passes > 1
+        same(r, a)
+
+
 def test_concurrent_submitters_and_poll(mcr):
     ps = stream(8, 24, paths=(1_000, 30_000))
     results = [None] * len(ps)
