@@ -572,8 +572,18 @@ int fetch_windows(const std::vector<WindowFetch>& w, float** hbuf, size_t* hcap,
 
 int try_resolve(const mcr_query* q, const Staged& st, const DevAccum& acc, int slot, int64_t R,
                 double* var, double* es, int64_t* tail) {
-  return resolve_quantile(q->paths, acc, slot, st.d.gather + (size_t)slot * kGatherCap, kGatherCap,
-                          R, var, es, tail, st.host ? nullptr : g_na_gather);
+  const float* gd = st.d.gather + (size_t)slot * kGatherCap;
+  // allocation-free mode: a synchronous copy into the static host buffer
+  if (!st.host) return resolve_quantile(q->paths, acc, slot, gd, kGatherCap, R, var, es, tail,
+                                        g_na_gather);
+  // otherwise on the query's own stream into the pinned window buffer (a copy
+  // on the null stream could wait behind another stream's kernels, such as a
+  // serving round's, when the two share a hardware queue)
+  if (!window_holds(q->paths, acc, slot, kGatherCap, R)) return 0;
+  const std::vector<WindowFetch> w{{gd, (int64_t)acc.gathered[slot], 0}};
+  if (fetch_windows(w, &g_ctx.hwin, &g_ctx.hwin_cap, g_ctx.stream) != MCR_OK) return -1;
+  resolve_window(q->paths, acc, slot, g_ctx.hwin, R, var, es, tail);
+  return 1;
 }
 
 int upload_windows(const Staged& st, const DevInfo& info) {
