@@ -576,16 +576,18 @@ int mcr_server::plan_round(uint64_t round, RoundRec* rr, std::vector<WorkItem>* 
               return a.first < b.first;
             });
   // Round length: round_ms while the admitted work is short; under a backlog
-  // it grows with it (an eighth of the backlog, at most 4 round_ms), so each
+  // it grows with it (an eighth of the backlog, at most 8 round_ms), so each
   // class launch of the round carries more work and its tail costs less.  A
   // 3000-query stream offered at 1600 QPS completed 1061 queries/s with 4 ms
   // rounds and 1418 / 1447 with 8 / 16 ms ones (p99 1.9 s -> 0.5 / 0.4 s),
   // while at 800 QPS, below saturation, the length changed nothing but p50
-  // (5.0 -> 5.9 ms at 16 ms).
+  // (5.0 -> 5.9 ms at 16 ms).  Past saturation (3200 QPS offered) 32 ms
+  // rounds complete 1747-1763 queries/s against 1637-1679 with 16 ms ones;
+  // below it nothing changes, as the backlog never asks for them.
   double backlog_units = 0.0;
   for (auto& o : open) backlog_units += o.first;
   const double backlog_ms = backlog_units * ms_per_unit / grid;
-  const double len_ms = std::min(4.0 * round_ms, std::max(round_ms, backlog_ms / 8.0));
+  const double len_ms = std::min(8.0 * round_ms, std::max(round_ms, backlog_ms / 8.0));
   double left = len_ms * grid / ms_per_unit;
   size_t k = open.size();
   std::vector<std::pair<Job*, uint64_t>> grant;
