@@ -70,9 +70,15 @@ def serve(mcr, ps, gaps_ms=2.0, seed=0, **opts):
 
 def test_served_results_equal_alone(mcr):
     ps = stream(1, 30)
+    t0 = time.perf_counter()
     out, st = serve(mcr, ps, round_ms=0.1)
+    elapsed_ms = (time.perf_counter() - t0) * 1e3
     assert st.completed == len(ps) and st.failed == 0
     assert st.cache_hits + st.cache_misses == len(ps)
+    # the worker's time per phase: each non-negative, together within the run
+    phases = [st.host_admit_ms, st.host_plan_ms, st.host_enqueue_ms, st.host_wait_ms,
+              st.host_finalize_ms]
+    assert all(v >= 0.0 for v in phases) and 0.0 < sum(phases) <= elapsed_ms
     assert st.cache_hits >= 1
     assert max(info.rounds for _, info in out) > 1  # some query spanned several rounds
     for p, (r, info) in zip(ps, out):
