@@ -582,8 +582,9 @@ int mcr_server::plan_round(uint64_t round, RoundRec* rr, std::vector<WorkItem>* 
   // rounds and 1418 / 1447 with 8 / 16 ms ones (p99 1.9 s -> 0.5 / 0.4 s),
   // while at 800 QPS, below saturation, the length changed nothing but p50
   // (5.0 -> 5.9 ms at 16 ms).  Past saturation (3200 QPS offered) 32 ms
-  // rounds complete 1747-1763 queries/s against 1637-1679 with 16 ms ones;
-  // below it nothing changes, as the backlog never asks for them.
+  // rounds complete 1747-1763 queries/s against 1637-1679 with 16 ms ones,
+  // and 64 ms ones 1633-1679 (the worker's staging of such rounds leaves the
+  // device idle); below it nothing changes, as the backlog never asks for them.
   double backlog_units = 0.0;
   for (auto& o : open) backlog_units += o.first;
   const double backlog_ms = backlog_units * ms_per_unit / grid;
