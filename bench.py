@@ -212,11 +212,12 @@ def batch_leg(mcr, np, wl):
     total = sum(p["paths"] for p in ps)
     best, tried = mcr.autotune_batch(ps)
     r0, o0 = mcr.run_batch(ps, use_graph=True)  # captures
-    dev, wall = [], []
+    dev, wall, host = [], [], []
     for _ in range(3):
         r1, o1 = mcr.run_batch(ps, use_graph=True)
         dev.append(o1.ms_device)
         wall.append(o1.ms_total)
+        host.append([o1.host_plan_ms, o1.host_stage_ms, o1.host_wait_ms, o1.host_finalize_ms])
     per, op = mcr.run_batch(ps, mode=mcr.BATCH_PER_QUERY)
     mism = [i for i in range(len(ps)) if not same_result(r1[i], per[i])]
     if mism:
@@ -236,10 +237,17 @@ def batch_leg(mcr, np, wl):
            "per_portfolio_device_ms": op.ms_device, "per_portfolio_wall_ms": op.ms_total,
            "per_portfolio_paths_per_s": total / (op.ms_device * 1e-3),
            "speedup_device": op.ms_device / ms_b, "speedup_wall": op.ms_total / wall_b,
+           # host time of a replayed call by phase (medians): plan the work
+           # items, stage and launch, wait for the device, finalise
+           "host_ms": dict(zip(("plan", "stage", "wait", "finalize"),
+                               (float(v) for v in np.median(np.array(host), axis=0)))),
            "results_equal_alone": True}
+    hm = res["host_ms"]
     print(f"# batch: persistent+graph {ms_b:.1f} ms device / {wall_b:.1f} ms wall, per-portfolio "
           f"{op.ms_device:.1f} ms device / {op.ms_total:.1f} ms wall ({total:.3g} paths); all "
-          f"{len(ps)} equal alone", file=sys.stderr, flush=True)
+          f"{len(ps)} equal alone; host plan/stage/wait/finalize "
+          f"{hm['plan']:.1f}/{hm['stage']:.1f}/{hm['wait']:.1f}/{hm['finalize']:.1f} ms",
+          file=sys.stderr, flush=True)
     return res
 
 

@@ -322,7 +322,9 @@ class BatchOpts(ctypes.Structure):
                 ("chunk_used", ctypes.c_int), ("order_used", ctypes.c_int),
                 ("graph_replayed", ctypes.c_int), ("work_items", ctypes.c_int64),
                 ("streams", ctypes.c_int), ("shapes", ctypes.c_int * 4),
-                ("streams_used", ctypes.c_int), ("big_shapes", ctypes.c_int * 4)]
+                ("streams_used", ctypes.c_int), ("big_shapes", ctypes.c_int * 4),
+                ("host_plan_ms", ctypes.c_double), ("host_stage_ms", ctypes.c_double),
+                ("host_wait_ms", ctypes.c_double), ("host_finalize_ms", ctypes.c_double)]
 
 
 def _queries(portfolios):

@@ -187,6 +187,11 @@ typedef struct mcr_batch_opts {
                         s > 0 = shape s - 1 (all shapes give identical results) */
   int streams_used;  /* out */
   int big_shapes[4]; /* out: shape used per range (0 = the measured default) */
+  /* out, persistent mode: host time of the call by phase (ms): planning the
+   * work items, staging the inputs (pinned copies, upload, graph lookup or
+   * capture and launch), waiting for the device, finalising the results
+   * (window copies and quantiles, any refinement) */
+  double host_plan_ms, host_stage_ms, host_wait_ms, host_finalize_ms;
 } mcr_batch_opts;
 
 /* Runs nq independent queries.  Persistent mode: every query uses the bf16x3
