@@ -162,6 +162,19 @@ def test_windows_holding_every_loss(mcr):
         same(r, alone(mcr, p))
 
 
+def test_four_var_levels_fit_their_window_buffers(mcr):
+    """Four VaR levels from the median to 99.9 % on up to 1e6 paths: the window
+    buffer sized for the widest expected window (the median's) holds every
+    level's losses, so each query resolves in one pass, equal to alone."""
+    ps = stream(11, 5, sizes=(12, 90, 300), paths=(200_000, 1_000_000))
+    for p in ps:
+        p["alphas"] = (0.5, 0.9, 0.99, 0.999)
+    out, _ = serve(mcr, ps)
+    for p, (r, _) in zip(ps, out):
+        assert r.var_passes == 1
+        same(r, alone(mcr, p))
+
+
 def test_window_overflow_falls_back_to_refinement(mcr):
     """Windows that hold every loss of 100k-150k paths overflow the served
     query's 2^16-loss window buffer (a single query keeps 2^18): the server
