@@ -12,7 +12,10 @@ Bars:
     second, else the first paths of the same query run alone);
   * the query stream (1000 queries, Poisson arrivals at 200 QPS, 30 % covariance
     reuse): every served result equals the query run alone, the same anchors
-    against FP64, and the latency p99 within the stated bound.
+    against FP64, and the latency p99 within the stated bound;
+  * at full size, the count and S histogram of one query equal the sums over
+    the same paths run as four ranges (fp32 and bf16x3 at n = 50, bf16x3 at
+    n = 600).
 bf16x3 tolerances (as in test_gpu_mfma.py): |count - count_fp64| <=
 max(3, 1e-4 count_fp64); VaR99 within 1e-4 relative of the FP64 order
 statistic."""
@@ -104,6 +107,29 @@ def test_fused_n50_full_size_fp32_equals_openmp_oracle(mcr, orc, wl):
     b = mcr.run(mu, sigma, C["x"], C["k"], C["paths"], C["path_seed"], precision=mcr.PREC_BF16X3)
     # bf16x3 on the same 1e8 paths: the relative bar on the exceedance probability
     assert abs(b.count - ref["count"]) <= 1e-4 * ref["count"], (b.count, ref["count"])
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
+def test_full_size_path_ranges_add_up(mcr, wl, precision):
+    """All 1e8 paths of the headline stream in one query equal the sum of the
+    same paths run as four uneven ranges (path_offset): the exceedance count
+    and the S histogram bin by bin -- the per-path results do not depend on
+    how the paths are grouped, which a run split over GPUs or resumed from a
+    saved tally relies on.  The large n = 600 workload (1e7 paths) is checked
+    the same way with bf16x3."""
+    prec = mcr.PREC_FP32 if precision == "fp32" else mcr.PREC_BF16X3
+    for C in (wl.FUSED_N50,) + ((wl.LARGE_N600,) if precision == "bf16x3" else ()):
+        mu, sigma = wl.model_of(mcr, C)
+        M = C["paths"]
+        whole = mcr.run(mu, sigma, C["x"], C["k"], M, C["path_seed"], precision=prec,
+                        want_hist=True)
+        cuts = [0, M // 7, M // 2, M - 12345, M]
+        parts = [mcr.run(mu, sigma, C["x"], C["k"], b - a, C["path_seed"], precision=prec,
+                         want_hist=True, path_offset=a) for a, b in zip(cuts, cuts[1:])]
+        assert all(r.kernel == whole.kernel for r in parts)
+        assert whole.count == sum(r.count for r in parts)
+        assert np.array_equal(whole.s_hist, np.sum([r.s_hist for r in parts], axis=0))
+        assert int(whole.s_hist.sum()) == M
 
 
 def test_near_flag_on_bf16x3_is_refused_and_auto_uses_fp32(mcr, wl):
