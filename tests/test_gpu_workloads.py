@@ -13,6 +13,8 @@ Bars:
   * the query stream (1000 queries, Poisson arrivals at 200 QPS, 30 % covariance
     reuse): every served result equals the query run alone, the same anchors
     against FP64, and the latency p99 within the stated bound;
+  * the fused n = 50 workload at full size with bf16x3: probability and
+    VaR99 / VaR99.9 within 1e-4 relative of the FP64 oracle (all 1e8 paths);
   * at full size, the count and S histogram of one query equal the sums over
     the same paths run as four ranges (fp32 and bf16x3 at n = 50, bf16x3 at
     n = 600).
@@ -107,6 +109,31 @@ def test_fused_n50_full_size_fp32_equals_openmp_oracle(mcr, orc, wl):
     b = mcr.run(mu, sigma, C["x"], C["k"], C["paths"], C["path_seed"], precision=mcr.PREC_BF16X3)
     # bf16x3 on the same 1e8 paths: the relative bar on the exceedance probability
     assert abs(b.count - ref["count"]) <= 1e-4 * ref["count"], (b.count, ref["count"])
+
+
+def test_fused_n50_full_size_bf16x3_within_bar_of_fp64(mcr, orc, wl):
+    """All 1e8 paths of the headline stream, bf16x3 MFMA against the FP64
+    oracle on the identical stream: the exceedance probability and VaR99 /
+    VaR99.9 each within 1e-4 relative (the precision bar), the VaR levels
+    against the FP64 losses' exact order statistics."""
+    C = wl.FUSED_N50
+    n, M = C["n"], C["paths"]
+    mu, sigma = wl.model_of(mcr, C)
+    r = mcr.run(mu, sigma, C["x"], C["k"], M, C["path_seed"], precision=mcr.PREC_BF16X3,
+                alphas=(0.99, 0.999))
+    L, *_ = mcr.factor(sigma)
+    ref = orc.simulate_f64(L, mu, np.full(n, 1.0 / n), C["x"], C["path_seed"], 0, M, C["k"],
+                           want_loss=True, nthreads=16)
+    idx = [rank_of(a, M) - 1 for a in (0.99, 0.999)]
+    part = np.partition(ref["loss"], idx)
+    v64 = [float(part[i]) for i in idx]
+    del part
+    print(f"fused n=50 bf16x3 vs FP64 over {M} paths: count {r.count} vs {ref['count']}, "
+          f"VaR99 {r.var[0]:.9g} vs {v64[0]:.9g}, VaR99.9 {r.var[1]:.9g} vs {v64[1]:.9g}")
+    assert r.kernel == "mfma"
+    assert abs(r.count - ref["count"]) <= 1e-4 * ref["count"]
+    for got, want in zip(r.var, v64):
+        assert abs(got - want) <= 1e-4 * abs(want)
 
 
 @pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
