@@ -15,5 +15,5 @@ export ASAN_OPTIONS="detect_leaks=0:halt_on_error=1:abort_on_error=1"
 export UBSAN_OPTIONS="halt_on_error=1:print_stacktrace=1"
 export LD_PRELOAD="$(gcc -print-file-name=libasan.so):$(gcc -print-file-name=libubsan.so)"
 cd "$ROOT"
-python -m pytest -q -p no:cacheprovider -m "not gpu" tests/test_io.py tests/test_abi.py \
-  tests/test_oracle_rng.py tests/test_oracle_paths.py "$@"
+python -m pytest -q -p no:cacheprovider -m "not gpu" tests/test_io.py tests/test_io_properties.py \
+  tests/test_abi.py tests/test_oracle_rng.py tests/test_oracle_paths.py "$@"
