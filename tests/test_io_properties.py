@@ -2,8 +2,10 @@
 config.yaml texts in block and flow style, with and without quotes and
 comments, read by the C loader (mcr_load_config) against the values written
 (PyYAML checks the text is YAML with those keys) and Python's own number
-parsing for the scalars; results-CSV
-rows with arbitrary comments and index names read back by Python's csv module.
+parsing for the scalars; results-CSV rows with arbitrary comments and index
+names read back by Python's csv module; arbitrary text into the config
+loader and arbitrary bytes into the binary readers, which must answer with a
+result or an error code (tests/sanitize.sh runs these under ASan/UBSan).
 
 What pins the loader's semantics is the reference's load_config
 (src/utilities/load_config.c:11-25 scalar conversions by atoi/atol/atof,
@@ -145,3 +147,71 @@ def test_csv_row_reads_back(mcr, comment, names):
     assert rows[0][3] == comment
     assert rows[0][15] == ";".join(f"{k}:{v}" for k, v in indices.items())
     assert rows[0][10] == "hip" and rows[0][18] == "500"
+
+
+yaml_ish = st.text(alphabet=st.sampled_from(list("abkxMN_:-[]{},'\"# \t\n0123456789.e")),
+                   max_size=200)
+
+
+@SETTINGS
+@given(text=st.one_of(yaml_ish, st.binary(max_size=200).map(lambda b: b.decode("latin-1"))))
+def test_config_loader_survives_any_text(mcr, tmp_path, text):
+    """Arbitrary bytes and YAML-like fragments (unbalanced brackets, stray
+    quotes, tabs, keys without values): the loader returns a configuration or
+    MCR_E_ARG, never anything else (and, under the sanitizer run, touches no
+    memory it does not own)."""
+    path = tmp_path / "fuzz.yaml"
+    path.write_bytes(text.encode("latin-1"))
+    try:
+        c = mcr.load_config(path)
+    except mcr.McrError as e:
+        assert e.code == mcr.MCR_E_ARG
+    else:
+        assert isinstance(c["models"], list) and isinstance(c["indices"], dict)
+
+
+@st.composite
+def bin_files(draw):
+    """A header that may lie about the payload, then a payload that may be
+    short, long or not numbers at all."""
+    import struct
+
+    rows = draw(st.integers(-3, 40))
+    cols = draw(st.integers(-3, 40))
+    kind = draw(st.sampled_from(["mu", "sigma", "params"]))
+    head = struct.pack("<i", rows) if kind == "mu" else (
+        struct.pack("<ii", rows, cols) if kind == "sigma" else b"")
+    body = draw(st.binary(max_size=8 * 60))
+    return kind, head + body
+
+
+@SETTINGS
+@given(bin_files())
+def test_binary_readers_survive_any_file(mcr, tmp_path, case):
+    """Headers that disagree with the file size, negative or zero counts, NaN
+    and out-of-range values: each reader returns its array or MCR_E_ARG /
+    MCR_E_IO-style errors, never a crash or a read past the file."""
+    kind, data = case
+    path = tmp_path / f"{kind}.bin"
+    path.write_bytes(data)
+    reader = {"mu": mcr.read_mu_bin, "sigma": mcr.read_sigma_bin,
+              "params": mcr.read_params_bin}[kind]
+    import math
+    import struct
+
+    try:
+        out = reader(path)
+    except mcr.McrError as e:
+        assert e.code < 0
+    else:
+        # what loaded is what the header announced, read from bytes the file
+        # has (trailing bytes are ignored, as the reference's fread does)
+        if kind == "mu":
+            (n,) = struct.unpack("<i", data[:4])
+            assert out.shape == (n,) and len(data) >= 4 + 8 * n
+        elif kind == "sigma":
+            r, c = struct.unpack("<ii", data[:8])
+            assert out.shape == (r, c) and len(data) >= 8 + 8 * r * c
+        else:
+            # the reference's range test lets a NaN through
+            assert len(data) >= 8 and (math.isnan(out) or 0.0 <= out <= 1.0)
