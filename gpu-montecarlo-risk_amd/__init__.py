@@ -374,7 +374,8 @@ def autotune_batch(portfolios):
 class ServerOpts(ctypes.Structure):
     _fields_ = [("hbm_budget", ctypes.c_int64), ("cache_entries", ctypes.c_int),
                 ("max_active", ctypes.c_int), ("round_ms", ctypes.c_double),
-                ("grid", ctypes.c_int), ("streams", ctypes.c_int)]
+                ("grid", ctypes.c_int), ("streams", ctypes.c_int),
+                ("age_limit_ms", ctypes.c_double)]
 
 
 class TicketInfo(ctypes.Structure):
@@ -426,10 +427,11 @@ class Server:
     their results.  Portfolios are dicts as for run_batch."""
 
     def __init__(self, device: int = 0, *, hbm_budget: int = 0, cache_entries: int = 0,
-                 max_active: int = 0, round_ms: float = 0.0, grid: int = 0, streams: int = 0):
+                 max_active: int = 0, round_ms: float = 0.0, grid: int = 0, streams: int = 0,
+                 age_limit_ms: float = 0.0):
         self._L = _server_lib()
         self._h = ctypes.c_void_p()
-        o = ServerOpts(hbm_budget, cache_entries, max_active, round_ms, grid, streams)
+        o = ServerOpts(hbm_budget, cache_entries, max_active, round_ms, grid, streams, age_limit_ms)
         _check(self._L.mcr_server_create(device, ctypes.byref(o), ctypes.byref(self._h)))
         self._na = {}
 

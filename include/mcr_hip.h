@@ -234,6 +234,11 @@ typedef struct mcr_server_opts {
   int grid;            /* persistent workgroups; 0 = one per CU */
   int streams;         /* concurrent streams a round's class launches are forked
                           over (1..2; 0 = 2, larger values are taken as 2) */
+  double age_limit_ms; /* rounds go to the queries with the least work left
+                          first; a query waiting longer than this since submit
+                          goes before them, oldest first (bounds the worst
+                          latency below saturation at some cost in p50/p99);
+                          0 = no limit */
 } mcr_server_opts;
 
 typedef struct mcr_ticket_info {

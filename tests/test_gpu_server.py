@@ -224,6 +224,22 @@ def test_rejects_unservable_queries(mcr):
             assert e.value.code == mcr.MCR_E_UNSUPPORTED
 
 
+def test_age_limit_changes_order_not_results(mcr):
+    """With an age limit every query passes at once (1 us), rounds plan the
+    queries oldest first instead of least work left first: each result still
+    equals the query alone.  A negative or NaN limit is refused."""
+    ps = stream(5, 20, paths=(2_000, 40_000))
+    out, st = serve(mcr, ps, gaps_ms=0.2, age_limit_ms=0.001)
+    assert st.completed == len(ps) and st.failed == 0
+    for p, (r, info) in zip(ps, out):
+        assert info.status == 0
+        same(r, alone(mcr, p))
+    for bad in (-1.0, float("nan")):
+        with pytest.raises(mcr.McrError) as e:
+            mcr.Server(age_limit_ms=bad)
+        assert e.value.code == mcr.MCR_E_ARG
+
+
 def test_forked_round_launches_equal_one_stream(mcr):
     """A round's class kernels forked over two streams give the results of
     one stream (same items, integer reductions)."""
