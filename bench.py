@@ -334,6 +334,18 @@ def large_leg(mcr, np, wl):
     res["tail_k"] = {"k": kt, "count": t.count, "p_hat": t.p_hat, "std_error": t.std_error,
                      "ci": [t.ci_lower, t.ci_upper], "kernel_ms": t.ms_kernel,
                      "count_equals_histogram_tail": t.count == int(h.s_hist[kt:].sum())}
+    # past 608 assets every precision runs the column-chunk fp32 kernel
+    C = wl.LARGEST_N1024
+    mu2, sigma2 = wl.model_of(mcr, C)
+    runs = [mcr.run(mu2, sigma2, C["x"], C["k"], C["paths"], C["path_seed"],
+                    precision=mcr.PREC_AUTO, alphas=C["alphas"]) for _ in range(3)]
+    ms = float(np.mean([r.ms_kernel for r in runs[1:]]))
+    r = runs[-1]
+    tf = C["n"] * (C["n"] + 1) * C["paths"] / (ms * 1e-3) / 1e12
+    res["n1024"] = {"workload": C["name"], "kernel": r.kernel, "kernel_ms": ms,
+                    "device_ms": r.ms_device, "paths_per_s": C["paths"] / (ms * 1e-3),
+                    "tflops": tf, "frac_fp32_peak": tf / PEAK_FP32_TFLOPS, "count": r.count,
+                    "p_hat": r.p_hat, "var99": r.var[0]}
     f, b = res["fp32"], res["bf16x3"]
     b["frac_bf16_peak"] = b["tflops"] / PEAK_BF16_TFLOPS
     f["frac_fp32_peak"] = f["tflops"] / PEAK_FP32_TFLOPS
@@ -343,7 +355,8 @@ def large_leg(mcr, np, wl):
     res["rel_diff_p_hat"] = abs(b["p_hat"] - f["p_hat"]) / f["p_hat"]
     res["rel_diff_var99"] = abs(b["var99"] - f["var99"]) / f["var99"]
     print(f"# n=600: fp32 {f['kernel_ms']:.2f} ms, bf16x3 {b['kernel_ms']:.2f} ms; tail k={kt}: "
-          f"P={t.p_hat:.5f} +- {t.std_error:.5f}", file=sys.stderr, flush=True)
+          f"P={t.p_hat:.5f} +- {t.std_error:.5f}; n=1024 (1e6 paths, {r.kernel}) {ms:.2f} ms, "
+          f"{tf:.1f} TFLOP/s", file=sys.stderr, flush=True)
     return res
 
 

@@ -35,12 +35,13 @@ HEADER_PATHS = [HEADER_PATH, os.path.join(os.path.dirname(_HERE), "include", "mc
 MCR_OK = 0
 MCR_E_ARG, MCR_E_UNSUPPORTED, MCR_E_DEVICE, MCR_E_NOMEM, MCR_E_FACTOR = -1, -2, -3, -4, -5
 MCR_MAX_ALPHA = 4
-MCR_MAX_N = 608
+MCR_MAX_N, MCR_MAX_N_BATCH = 1024, 608
 PREC_FP32, PREC_BF16X3, PREC_AUTO = 0, 1, 2
 KERNEL_AUTO, KERNEL_VALU_REG, KERNEL_VALU_TILE, KERNEL_MFMA, KERNEL_MFMA_LDS = 0, 1, 2, 3, 4
+KERNEL_VALU_CHUNK = 5
 F_VAR, F_NEAR, F_SHIST, F_TEST_NARROW, F_TEST_WIDE = 0x1, 0x2, 0x4, 0x100, 0x200
 KERNEL_NAMES = {KERNEL_VALU_REG: "valu_reg", KERNEL_VALU_TILE: "valu_tile", KERNEL_MFMA: "mfma",
-                KERNEL_MFMA_LDS: "mfma_lds"}
+                KERNEL_MFMA_LDS: "mfma_lds", KERNEL_VALU_CHUNK: "valu_chunk"}
 
 
 class McrError(RuntimeError):
@@ -501,7 +502,7 @@ def prepare(mu, sigma, x: float, weights=None) -> dict:
     q = Query()
     q.n, q.mu, q.sigma, q.ld_sigma = n, _ptr(mu, _f64p), _ptr(sigma, _f64p), n
     q.weights, q.x, q.k, q.paths = _ptr(w, _f64p), float(x), 1, 1
-    q.precision, q.kernel = PREC_FP32, KERNEL_VALU_TILE
+    q.precision, q.kernel = PREC_FP32, KERNEL_AUTO
     L64 = np.zeros((n, n), dtype=np.float64)
     L32 = np.zeros((n, n), dtype=np.float32)
     mu32 = np.zeros(n, dtype=np.float32)

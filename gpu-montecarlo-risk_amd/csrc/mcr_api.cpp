@@ -216,6 +216,9 @@ size_t layout(const Plan& p, char* base, DevBuffers* b) {
   return c.off;
 }
 
+static_assert(MCR_MAX_N == kChunkMaxN && MCR_MAX_N_BATCH == kMaxN,
+              "public size limits follow the kernels'");
+
 int choose_kernel(const mcr_query* q, Plan* p) {
   const int n = q->n;
   int k = q->kernel;
@@ -224,13 +227,19 @@ int choose_kernel(const mcr_query* q, Plan* p) {
   // error of several ulp, so a 1-ulp window there would name no definite set.
   const bool near = (q->flags & MCR_F_NEAR) != 0;
   if (k == MCR_KERNEL_AUTO) {
-    if (q->precision == MCR_PREC_FP32 || (q->precision == MCR_PREC_AUTO && near)) {
-      k = n <= kRegMaxN ? MCR_KERNEL_VALU_REG : MCR_KERNEL_VALU_TILE;
-    } else if (q->precision == MCR_PREC_BF16X3 || q->precision == MCR_PREC_AUTO) {
-      k = n <= 32 * kMfmaMaxT - 1 ? MCR_KERNEL_MFMA : MCR_KERNEL_MFMA_LDS;
-    } else {
+    if (q->precision != MCR_PREC_FP32 && q->precision != MCR_PREC_BF16X3 &&
+        q->precision != MCR_PREC_AUTO) {
       set_error("unknown precision %d", q->precision);
       return MCR_E_ARG;
+    }
+    if (n > kMaxN) {
+      // the column-chunk fp32 kernel is the only one past the tile and MFMA
+      // kernels' 608 (fp32 error is no larger than the bf16x3 products')
+      k = MCR_KERNEL_VALU_CHUNK;
+    } else if (q->precision == MCR_PREC_FP32 || (q->precision == MCR_PREC_AUTO && near)) {
+      k = n <= kRegMaxN ? MCR_KERNEL_VALU_REG : MCR_KERNEL_VALU_TILE;
+    } else {
+      k = n <= 32 * kMfmaMaxT - 1 ? MCR_KERNEL_MFMA : MCR_KERNEL_MFMA_LDS;
     }
   }
   if (k == MCR_KERNEL_VALU_REG && n > kRegMaxN) {
@@ -241,9 +250,14 @@ int choose_kernel(const mcr_query* q, Plan* p) {
     set_error("MFMA kernel supports n <= %d (n = %d)", 32 * kMfmaMaxT - 1, n);
     return MCR_E_UNSUPPORTED;
   }
-  if (k < MCR_KERNEL_VALU_REG || k > MCR_KERNEL_MFMA_LDS) {
+  if (k < MCR_KERNEL_VALU_REG || k > MCR_KERNEL_VALU_CHUNK) {
     set_error("unknown kernel %d", k);
     return MCR_E_ARG;
+  }
+  if ((k == MCR_KERNEL_VALU_TILE || k == MCR_KERNEL_MFMA_LDS) && n > kMaxN) {
+    set_error("%s kernel supports n <= %d (n = %d); MCR_KERNEL_VALU_CHUNK takes n <= %d",
+              k == MCR_KERNEL_VALU_TILE ? "tile" : "LDS MFMA", kMaxN, n, kChunkMaxN);
+    return MCR_E_UNSUPPORTED;
   }
   if (near && (k == MCR_KERNEL_MFMA || k == MCR_KERNEL_MFMA_LDS)) {
     set_error("MCR_F_NEAR needs an fp32 kernel (precision MCR_PREC_FP32 or MCR_PREC_AUTO); "
@@ -331,6 +345,10 @@ int stage_query(const mcr_query* q, Staged* st, bool want_var_windows, bool noal
   const size_t dev_bytes = layout(p, nullptr, &st->d);
   const size_t n = (size_t)q->n;
   if (noalloc) {
+    if (q->n > kMaxN) {
+      set_error("the allocation-free entry point takes n <= %d (n = %d)", kMaxN, q->n);
+      return MCR_E_UNSUPPORTED;
+    }
     if (dev_bytes > kScratchBytes) {
       set_error("query needs %zu bytes of device scratch, more than the %zu the "
                 "allocation-free entry point holds", dev_bytes, kScratchBytes);
@@ -1026,6 +1044,10 @@ int check_batch_query(const mcr_query* q, int i) {
               q->precision);
     return MCR_E_UNSUPPORTED;
   }
+  if (q->n > kMaxN) {
+    set_error("query %d: batches take n <= %d (n = %d)", i, kMaxN, q->n);
+    return MCR_E_UNSUPPORTED;
+  }
   if (q->S_out || q->loss_out || (q->flags & (MCR_F_SHIST | MCR_F_NEAR))) {
     set_error("query %d: per-path outputs, S histograms and near counts are single-query options",
               i);
@@ -1530,7 +1552,7 @@ int mcr_factor(int n, const double* sigma, int ld_sigma, double* L_out, mcr_fact
   q.ld_sigma = ld_sigma;
   q.paths = 1;
   q.precision = MCR_PREC_FP32;
-  q.kernel = MCR_KERNEL_VALU_TILE;
+  q.kernel = MCR_KERNEL_AUTO;
   mcr_model_export ex = {};
   ex.L64 = L_out;
   std::lock_guard<std::mutex> lk(g_mu);

@@ -58,19 +58,23 @@ __device__ double block_sum_f64(double v, double* red) {
 // is not above n eps (A_jj + delta).  Returns false (and records attempts =
 // -1) for non-finite A.
 constexpr int kPanel = 16;
-constexpr int kPanelLd = (kMaxN + 3) / 4 * 4 + 4;  // rows + tile overrun, multiple of 4
-
+// MAXN: the largest n the kernel is built for (kMaxN: 78 KiB of LDS; the
+// column-chunk queries' kChunkMaxN: 129 KiB); a panel row per thread.
+template <int MAXN>
 struct FactorShared {
-  __attribute__((aligned(16))) double panel[kPanel * kPanelLd];  // [c * kPanelLd + r]
+  static constexpr int kLd = (MAXN + 3) / 4 * 4 + 4;  // rows + tile overrun, multiple of 4
+  __attribute__((aligned(16))) double panel[kPanel * kLd];  // [c * kLd + r]
   double pivot[2][kPanel];
   double adiag[kPanel];  // A_jj + delta of the panel's columns (pivot tolerance)
   double red[16];
 };
 
-__device__ bool cholesky_body(const FactorArgs& a, FactorShared& sh, int* attempts_out,
+template <int MAXN>
+__device__ bool cholesky_body(const FactorArgs& a, FactorShared<MAXN>& sh, int* attempts_out,
                               int* clamped_out, int* fail_out, double* delta_out) {
   const int n = a.n;
-  const int tid = threadIdx.x, nthr = blockDim.x;  // nthr >= kMaxN: every panel row has an owner
+  const int tid = threadIdx.x, nthr = blockDim.x;  // nthr >= MAXN: every panel row has an owner
+  constexpr int kPanelLd = FactorShared<MAXN>::kLd;
   const int lane = tid & 63, wave = tid >> 6, nwave = nthr >> 6;
   const double* __restrict__ A = a.A;
   double* __restrict__ L = a.L;
@@ -261,7 +265,8 @@ __device__ void moments_body(const FactorArgs& a, double* red, int attempts, int
 }
 
 
-__device__ void factor_body(const FactorArgs& a, FactorShared& sh) {
+template <int MAXN>
+__device__ void factor_body(const FactorArgs& a, FactorShared<MAXN>& sh) {
   const int tid = threadIdx.x;
 
   // Clear the query's accumulators.
@@ -301,20 +306,25 @@ __device__ void factor_body(const FactorArgs& a, FactorShared& sh) {
   moments_body(a, sh.red, attempts, clamped, last_fail, delta);
 }
 
+template <int MAXN>
 __global__ __launch_bounds__(1024) void k_factor(FactorArgs a) {
-  __shared__ FactorShared sh;
+  __shared__ FactorShared<MAXN> sh;
   factor_body(a, sh);
 }
 
-// One workgroup per query of a batch.
+// One workgroup per query of a batch (n <= kMaxN).
 __global__ __launch_bounds__(1024) void k_factor_batch(const FactorArgs* fas) {
-  __shared__ FactorShared sh;
+  __shared__ FactorShared<kMaxN> sh;
   const FactorArgs a = fas[blockIdx.x];
   factor_body(a, sh);
 }
 
 hipError_t launch_factor(const FactorArgs& a, hipStream_t stream) {
-  hipLaunchKernelGGL(k_factor, dim3(1), dim3(1024), 0, stream, a);
+  if (a.n < 1 || a.n > kChunkMaxN) return hipErrorInvalidValue;
+  if (a.n <= kMaxN)
+    hipLaunchKernelGGL(k_factor<kMaxN>, dim3(1), dim3(1024), 0, stream, a);
+  else
+    hipLaunchKernelGGL(k_factor<kChunkMaxN>, dim3(1), dim3(1024), 0, stream, a);
   return hipGetLastError();
 }
 
@@ -567,6 +577,161 @@ __global__ __launch_bounds__(64 * NW) void k_valu_tile(PathArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Column-chunk kernel for n up to kChunkMaxN (and any smaller n): 64 paths per
+// workgroup, 16 waves.  The paths' normals no longer fit the LDS whole, so the
+// columns are taken in chunks of kChunkBlocks 4-column blocks (512 columns,
+// 128 KiB): the waves generate a chunk's normals into LDS, then every wave
+// advances all of its 8-row register blocks (snake order, the loss row one
+// more block) over the chunk's columns.  Each wave's blocks stay in registers
+// across the chunks, so row i still sees mu_i, then z_0 .. z_i in order and
+// the results equal the tile and register kernels' bit for bit.
+// ---------------------------------------------------------------------------
+constexpr int kChunkBlocks = 128;  // 4-column blocks per chunk (even: a diagonal
+                                   // block's two column blocks share a chunk)
+constexpr int kChunkNW = 16;
+constexpr int kChunkMB = (kChunkMaxN / kTileRows + 1 + kChunkNW - 1) / kChunkNW;  // blocks per wave
+
+__global__ __launch_bounds__(64 * kChunkNW) void k_valu_chunk(PathArgs a) {
+  constexpr int NW = kChunkNW;
+  __shared__ float4 zs[kChunkBlocks * 64];  // [block - chunk start][lane]
+  __shared__ int sS[NW][64];
+  __shared__ float sLoss[64];
+  __shared__ unsigned sh_hist[kChunkMaxN + 1];
+  const int n = a.n;
+  const int nb = (n + 3) >> 2;
+  const int nrb = (n + kTileRows - 1) / kTileRows;  // row blocks; block nrb is the loss row
+  const int ldl = nrb * kTileRows;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const bool want_hist = a.s_hist != nullptr;
+  if (want_hist) {
+    for (int i = threadIdx.x; i <= n; i += blockDim.x) sh_hist[i] = 0u;
+  }
+  MCR_CONSTP(float) L = cview(a.L32);
+  MCR_CONSTP(float) mu = cview(a.mu32);
+  MCR_CONSTP(float) v = cview(a.v32);
+  const float mp32 = (float)cview(a.info)->mp;
+  const Key2 key = key_of(a.seed);
+  const bool want_near = a.thr_lo < a.thr_hi;
+  LaneTally t;
+  zero_tally(t);
+
+  for (uint64_t tile = blockIdx.x; tile * kTilePaths < a.npaths; tile += gridDim.x) {
+    const uint64_t q = tile * kTilePaths + lane;
+    const uint64_t path = a.path0 + q;
+    // this wave's k-th row block: rb = k NW + (k odd ? NW - 1 - wave : wave)
+    float acc[kChunkMB][kTileRows];
+#pragma unroll
+    for (int k = 0; k < kChunkMB; ++k) {
+      const int rb = k * NW + ((k & 1) ? NW - 1 - wave : wave);
+#pragma unroll
+      for (int r = 0; r < kTileRows; ++r) {
+        const int i = rb * kTileRows + r;
+        acc[k][r] = rb < nrb ? (i < n ? mu[i] : 0.0f) : (rb == nrb && r == 0 ? mp32 : 0.0f);
+      }
+    }
+    for (int jb0 = 0; jb0 < nb; jb0 += kChunkBlocks) {
+      const int jb1 = min(nb, jb0 + kChunkBlocks);
+      for (int b = jb0 + wave; b < jb1; b += NW) {
+        float z[4];
+        normals4(key, path, (uint32_t)b, z);
+        zs[(b - jb0) * 64 + lane] = make_float4(z[0], z[1], z[2], z[3]);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < kChunkMB; ++k) {
+        const int rb = k * NW + ((k & 1) ? NW - 1 - wave : wave);
+        if (rb > nrb) continue;
+        if (rb == nrb) {
+          // loss row: g = w^T mu + sum_j v_j z_j, j ascending
+          float g = acc[k][0];
+          for (int jb = jb0; jb < jb1; ++jb) {
+            const float4 z4 = zs[(jb - jb0) * 64 + lane];
+            const int rem = n - 4 * jb;  // >= 4 except in the last block
+            g = __builtin_fmaf(v[4 * jb + 0], z4.x, g);
+            if (rem > 1) g = __builtin_fmaf(v[4 * jb + 1], z4.y, g);
+            if (rem > 2) g = __builtin_fmaf(v[4 * jb + 2], z4.z, g);
+            if (rem > 3) g = __builtin_fmaf(v[4 * jb + 3], z4.w, g);
+          }
+          acc[k][0] = g;
+          continue;
+        }
+        const int i0 = rb * kTileRows;
+        const int J = i0 / 4;  // column blocks left of the diagonal block
+        MCR_CONSTP(float) Lb = L + l32_offset(ldl, i0, 0);
+        const int je = min(jb1, J);
+        for (int jb = jb0; jb < je; ++jb) {
+          const float4 z4 = zs[(jb - jb0) * 64 + lane];
+          MCR_CONSTP(float) lv = Lb + 32 * jb;
+#pragma unroll
+          for (int r = 0; r < kTileRows; ++r) {
+            acc[k][r] = __builtin_fmaf(lv[4 * r + 0], z4.x, acc[k][r]);
+            acc[k][r] = __builtin_fmaf(lv[4 * r + 1], z4.y, acc[k][r]);
+            acc[k][r] = __builtin_fmaf(lv[4 * r + 2], z4.z, acc[k][r]);
+            acc[k][r] = __builtin_fmaf(lv[4 * r + 3], z4.w, acc[k][r]);
+          }
+        }
+        if (J >= jb0 && J < jb1) {
+          // diagonal 8x8 block (column blocks J and J+1, both in this chunk)
+          float zd[8];
+          const float4 z4 = zs[(J - jb0) * 64 + lane];
+          zd[0] = z4.x; zd[1] = z4.y; zd[2] = z4.z; zd[3] = z4.w;
+          if (J + 1 < nb) {
+            const float4 z5 = zs[(J + 1 - jb0) * 64 + lane];
+            zd[4] = z5.x; zd[5] = z5.y; zd[6] = z5.z; zd[7] = z5.w;
+          } else {
+            zd[4] = zd[5] = zd[6] = zd[7] = 0.0f;
+          }
+#pragma unroll
+          for (int r = 0; r < kTileRows; ++r) {
+#pragma unroll
+            for (int c = 0; c <= r; ++c)
+              acc[k][r] = __builtin_fmaf(Lb[32 * (J + (c >> 2)) + 4 * r + (c & 3)], zd[c], acc[k][r]);
+          }
+        }
+      }
+      __syncthreads();  // the chunk's normals are read; the next chunk may overwrite them
+    }
+
+    int S = 0;
+    bool near = false;
+#pragma unroll
+    for (int k = 0; k < kChunkMB; ++k) {
+      const int rb = k * NW + ((k & 1) ? NW - 1 - wave : wave);
+      if (rb == nrb) sLoss[lane] = -acc[k][0];
+      if (rb >= nrb) continue;
+#pragma unroll
+      for (int r = 0; r < kTileRows; ++r) {
+        if (rb * kTileRows + r < n) {
+          S += (acc[k][r] < a.thr) ? 1 : 0;
+          if (want_near) near |= (acc[k][r] >= a.thr_lo) & (acc[k][r] <= a.thr_hi);
+        }
+      }
+    }
+    sS[wave][lane] = S | (near ? (1 << 30) : 0);
+    __syncthreads();
+    if (wave == 0 && q < a.npaths) {
+      int tot = 0;
+      bool nr = false;
+#pragma unroll
+      for (int w = 0; w < NW; ++w) {
+        const int e = sS[w][lane];
+        tot += e & 0xffff;
+        nr |= (e >> 30) & 1;
+      }
+      tally_path(a, t, q, tot, sLoss[lane], nr, want_hist ? sh_hist : nullptr);
+    }
+    __syncthreads();
+  }
+  flush_tally(a, t);
+  if (want_hist) {
+    __syncthreads();
+    for (int i = threadIdx.x; i <= n; i += blockDim.x)
+      if (sh_hist[i]) atomicAdd(&a.s_hist[i], (unsigned long long)sh_hist[i]);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Debug: the normal stream itself, one path per lane.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_normals(uint64_t seed, uint64_t path0, uint64_t npaths,
@@ -647,6 +812,13 @@ hipError_t launch_paths(int kernel, const PathArgs& a, int num_cus, hipStream_t 
       hipLaunchKernelGGL(k_valu_tile<8>, dim3((unsigned)blocks), dim3(512), lds, stream, a);
     else
       hipLaunchKernelGGL(k_valu_tile<4>, dim3((unsigned)blocks), dim3(256), lds, stream, a);
+    return hipGetLastError();
+  }
+  if (kernel == 5) {
+    if (a.n < 1 || a.n > kChunkMaxN) return hipErrorInvalidValue;
+    // one 1024-thread workgroup per CU (139.5 KiB of LDS)
+    const uint64_t blocks = min_u64((a.npaths + kTilePaths - 1) / kTilePaths, (uint64_t)num_cus);
+    hipLaunchKernelGGL(k_valu_chunk, dim3((unsigned)blocks), dim3(64 * kChunkNW), 0, stream, a);
     return hipGetLastError();
   }
   if (kernel == 4) {

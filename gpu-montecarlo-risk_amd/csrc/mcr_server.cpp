@@ -236,6 +236,10 @@ int check_servable(const mcr_query* q) {
     set_error("served queries run the bf16x3 MFMA kernels (precision %d)", q->precision);
     return MCR_E_UNSUPPORTED;
   }
+  if (q->n > kMaxN) {
+    set_error("served queries take n <= %d (n = %d)", kMaxN, q->n);
+    return MCR_E_UNSUPPORTED;
+  }
   if (q->S_out || q->loss_out || (q->flags & (MCR_F_SHIST | MCR_F_NEAR))) {
     set_error("per-path outputs, S histograms and near counts are single-query options");
     return MCR_E_UNSUPPORTED;

@@ -38,7 +38,8 @@ extern "C" {
 #define MCR_E_FACTOR -5      /* covariance not factorable (non-finite input) */
 
 #define MCR_MAX_ALPHA 4
-#define MCR_MAX_N 608 /* largest asset count any kernel accepts */
+#define MCR_MAX_N 1024      /* largest asset count of a single query (mcr_run) */
+#define MCR_MAX_N_BATCH 608 /* largest in batches, the serving loop and mcr_run_noalloc */
 
 /* precision / kernel family */
 #define MCR_PREC_FP32 0   /* fused fp32 VALU kernels: bit-exact with the C mirror */
@@ -51,6 +52,9 @@ extern "C" {
 #define MCR_KERNEL_VALU_TILE 2 /* 64 paths per workgroup, normals in LDS, n <= 608 */
 #define MCR_KERNEL_MFMA 3      /* 64 paths per wave on 32x32x16 bf16 MFMA, n <= 127 */
 #define MCR_KERNEL_MFMA_LDS 4  /* 64 paths per workgroup, normals staged in LDS, n <= 608 */
+#define MCR_KERNEL_VALU_CHUNK 5 /* fp32, 64 paths per workgroup, normals in 512-column LDS
+                                   chunks, any n <= 1024; the only kernel for n > 608, which
+                                   every precision then runs */
 
 /* flags */
 #define MCR_F_VAR 0x1   /* portfolio-loss quantiles (VaR) and tail sums at alpha[] */
@@ -65,7 +69,8 @@ extern "C" {
 #define MCR_F_TEST_WIDE 0x200
 
 typedef struct mcr_query {
-  int n;                /* assets, 1 <= n <= MCR_MAX_N */
+  int n;                /* assets, 1 <= n <= MCR_MAX_N (MCR_MAX_N_BATCH in batches,
+                           the serving loop and mcr_run_noalloc) */
   const double *mu;     /* [n] mean daily returns */
   const double *sigma;  /* [n * ld_sigma] covariance, row-major */
   int ld_sigma;         /* row stride of sigma (>= n) */
@@ -143,9 +148,9 @@ int mcr_run(const mcr_query *q, mcr_result *r);
  * 16 MiB array of static device memory that the library holds on the device
  * until the process ends, and the host side uses static buffers: no
  * hipMalloc, hipFree, pinned or heap allocation.  Needs mcr_init first;
- * fails with MCR_E_UNSUPPORTED when the query's buffers exceed the array
- * (they do not for n <= MCR_MAX_N; per-path outputs are staged 2^18 paths
- * at a time).  Calls share the one array, so calls that overlap on one
+ * fails with MCR_E_UNSUPPORTED when n > MCR_MAX_N_BATCH or the query's
+ * buffers exceed the array (they do not for n <= MCR_MAX_N_BATCH; per-path
+ * outputs are staged 2^18 paths at a time).  Calls share the one array, so calls that overlap on one
  * device race: they are serialised with mcr_run by the library's lock. */
 int mcr_run_noalloc(const mcr_query *q, mcr_result *r);
 

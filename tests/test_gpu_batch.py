@@ -186,3 +186,10 @@ def test_batch_rejects_fp32_queries(mcr):
     with pytest.raises(mcr.McrError) as e:
         mcr.run_batch(ps)
     assert e.value.code == mcr.MCR_E_UNSUPPORTED
+
+
+def test_batch_rejects_more_than_608_assets(mcr):
+    ps = portfolios(8, sizes=(10, 609), paths=[100, 100])
+    with pytest.raises(mcr.McrError) as e:
+        mcr.run_batch(ps)
+    assert e.value.code == mcr.MCR_E_UNSUPPORTED

@@ -70,9 +70,9 @@ def test_c_abi_argument_errors(mcr):
     with pytest.raises(mcr.McrError) as e:
         mcr.run(mu, sigma, 0.02, 1, 10, 1, kernel=mcr.KERNEL_MFMA, alphas=(1.5,))
     assert e.value.code == mcr.MCR_E_ARG
-    big = np.eye(700) * 1e-4
+    big = np.eye(1025) * 1e-4
     with pytest.raises(mcr.McrError) as e:
-        mcr.run(np.zeros(700), big, 0.02, 1, 10, 1)
+        mcr.run(np.zeros(1025), big, 0.02, 1, 10, 1)
     assert e.value.code == mcr.MCR_E_UNSUPPORTED
     with pytest.raises(mcr.McrError) as e:
         mcr.run(np.zeros(100), np.eye(100) * 1e-4, 0.02, 1, 10, 1, kernel=mcr.KERNEL_VALU_REG)

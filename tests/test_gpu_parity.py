@@ -131,13 +131,65 @@ def test_tile_kernel_exact(mcr, orc, n):
     assert r.kernel == "valu_tile"
 
 
+@pytest.mark.parametrize("n", [1, 5, 9, 50, 129, 257, 600, 608, 609, 700, 1000, 1024])
+def test_chunk_kernel_exact(mcr, orc, n):
+    """The column-chunk kernel (any n <= 1024; the only one past 608): per-path
+    S and loss bits, count, near-threshold count and S histogram equal the
+    oracle's.  n = 609-1024 covers one and two 512-column chunks, a last
+    chunk of one 4-column block (n = 1024 - 3 .. 1024 ends on block 255) and
+    n not a multiple of 4 or 8."""
+    M = 20000 if n <= 129 else (4000 if n <= 608 else 1500)
+    T = max(4000, 2 * n)
+    mu, sigma = factor_like(n, min(12, n), 31 + n, T=T)
+    model, _ = device_model(mcr, orc, mu, sigma, 0.02)
+    k = max(1, n // 5)
+    r = mcr.run(mu, sigma, 0.02, k, M, 2024, kernel=mcr.KERNEL_VALU_CHUNK, want_S=True,
+                want_loss=True, near=True, want_hist=True)
+    ref = orc.simulate_f32(model, 2024, 0, M, k, want_S=True, want_loss=True, want_hist=True,
+                           nthreads=8)
+    assert r.kernel == "valu_chunk"
+    bad = np.nonzero(r.S != ref["S"])[0]
+    assert bad.size == 0, f"{bad.size} paths differ, first {bad[:5]}"
+    assert np.array_equal(r.loss.view(np.uint32), ref["loss"].view(np.uint32))
+    assert r.count == ref["count"] and r.near_threshold == ref["near"]
+    assert np.array_equal(r.s_hist, ref["hist"])
+
+
+def test_chunk_and_tile_kernels_agree_at_scale(mcr):
+    """Size-independent property at the n = 600 workload's shape (1e6
+    paths): the column-chunk and tile kernels compute identical per-path
+    results, so their counts, tails and quantiles are identical."""
+    mu, sigma = factor_like(600, 20, 6000, T=4000)
+    a = mcr.run(mu, sigma, 0.02, 597, 1_000_000, 5, kernel=mcr.KERNEL_VALU_TILE, alphas=(0.99,))
+    b = mcr.run(mu, sigma, 0.02, 597, 1_000_000, 5, kernel=mcr.KERNEL_VALU_CHUNK, alphas=(0.99,))
+    assert a.count == b.count and a.var == b.var and a.tail_sum_fixed == b.tail_sum_fixed
+
+
+def test_past_608_assets_every_precision_runs_the_chunk_kernel(mcr):
+    """Above 608 assets every precision (and the automatic choice) runs the
+    column-chunk kernel and gives the same result; naming the tile or LDS MFMA
+    kernel there is refused, as is n = 1025."""
+    mu, sigma = factor_like(800, 10, 8000, T=2000)
+    rs = [mcr.run(mu, sigma, 0.02, 80, 5000, 3, precision=p, alphas=(0.99,))
+          for p in (mcr.PREC_FP32, mcr.PREC_BF16X3, mcr.PREC_AUTO)]
+    assert all(r.kernel == "valu_chunk" for r in rs)
+    assert len({(r.count, r.var[0]) for r in rs}) == 1
+    for kernel in (mcr.KERNEL_VALU_TILE, mcr.KERNEL_MFMA_LDS, mcr.KERNEL_MFMA):
+        with pytest.raises(mcr.McrError) as e:
+            mcr.run(mu, sigma, 0.02, 80, 100, 3, kernel=kernel)
+        assert e.value.code == mcr.MCR_E_UNSUPPORTED
+    with pytest.raises(mcr.McrError) as e:
+        mcr.run(np.zeros(1025), np.eye(1025) * 1e-4, 0.02, 1, 10, 1)
+    assert e.value.code == mcr.MCR_E_UNSUPPORTED
+
+
 @pytest.mark.parametrize("M", [1, 63, 64, 65, 257, 1000])
-@pytest.mark.parametrize("kernel", [1, 2])
+@pytest.mark.parametrize("kernel", [1, 2, 5])
 def test_ragged_path_counts(mcr, orc, M, kernel):
     _exact_case(mcr, orc, 20, M, 5, kernel)
 
 
-@pytest.mark.parametrize("kernel", [1, 2])
+@pytest.mark.parametrize("kernel", [1, 2, 5])
 def test_path_ids_cross_32_bits_and_large_seed(mcr, orc, kernel):
     _exact_case(mcr, orc, 11, 5000, 3, kernel, seed=2**64 - 1, path_offset=2**32 - 2500)
 

@@ -217,7 +217,8 @@ def test_concurrent_submitters_and_poll(mcr):
 def test_rejects_unservable_queries(mcr):
     p = stream(9, 1, sizes=(10,))[0]
     with mcr.Server() as srv:
-        for bad in ({"precision": mcr.PREC_FP32}, {"extra_flags": mcr.F_NEAR}):
+        big = {"mu": np.zeros(700), "sigma": np.eye(700) * 1e-4, "weights": None}
+        for bad in ({"precision": mcr.PREC_FP32}, {"extra_flags": mcr.F_NEAR}, big):
             q = dict(p, **bad)
             with pytest.raises(mcr.McrError) as e:
                 srv.submit(q)

@@ -28,7 +28,7 @@ def oracle_quantiles(orc, mcr, mu, sigma, x, w, seed, M, alphas, path_offset=0):
     return out
 
 
-@pytest.mark.parametrize("kernel", [1, 2])
+@pytest.mark.parametrize("kernel", [1, 2, 5])
 @pytest.mark.parametrize("weights", ["equal", "dirichlet"])
 def test_var_exact(mcr, orc, kernel, weights):
     n, M, seed = 24, 200_000, 99
@@ -44,7 +44,7 @@ def test_var_exact(mcr, orc, kernel, weights):
     assert r.var_passes == 1
 
 
-@pytest.mark.parametrize("kernel", [1, 2])
+@pytest.mark.parametrize("kernel", [1, 2, 5])
 def test_var_refinement_path(mcr, orc, kernel):
     n, M, seed = 10, 50_000, 3
     mu, sigma = factor_like(n, 2, 22)
@@ -88,6 +88,23 @@ def test_mfma_var_independent_of_window_width(mcr, n):
     assert list(a.var) == list(b.var)
     assert list(a.tail_sum_fixed) == list(b.tail_sum_fixed)
     assert list(a.es) == list(b.es)
+
+
+@pytest.mark.parametrize("refine", [False, True])
+def test_var_exact_past_608_assets(mcr, orc, refine):
+    """n = 1000 (two 512-column chunks of the column-chunk kernel, chosen
+    automatically at every precision): VaR, tail sums and ES equal the
+    oracle's, in one pass and through the refinement passes."""
+    n, M, seed = 1000, 6000, 17
+    mu, sigma = factor_like(n, 12, 23, T=2500)
+    w = np.random.default_rng(6).dirichlet(np.ones(n))
+    alphas = (0.9, 0.99)
+    r = mcr.run(mu, sigma, 0.02, 100, M, seed, weights=w, precision=mcr.PREC_BF16X3,
+                alphas=alphas, extra_flags=mcr.F_TEST_NARROW if refine else 0)
+    assert r.kernel == "valu_chunk" and (r.var_passes > 1) == refine
+    ref = oracle_quantiles(orc, mcr, mu, sigma, 0.02, w, seed, M, alphas)
+    for i, (var, fixed, es) in enumerate(ref):
+        assert r.var[i] == var and r.tail_sum_fixed[i] == fixed and r.es[i] == es
 
 
 @pytest.mark.parametrize("M", [1, 2, 10, 100])

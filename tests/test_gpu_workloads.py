@@ -17,7 +17,9 @@ Bars:
     VaR99 / VaR99.9 within 1e-4 relative of the FP64 oracle (all 1e8 paths);
   * at full size, the count and S histogram of one query equal the sums over
     the same paths run as four ranges (fp32 and bf16x3 at n = 50, bf16x3 at
-    n = 600).
+    n = 600, fp32 on the column-chunk kernel at n = 1024);
+  * the n = 1024 workload (column-chunk kernel, chosen for every precision)
+    within the bf16x3 tolerances of the FP64 oracle on the identical stream.
 bf16x3 tolerances (as in test_gpu_mfma.py): |count - count_fp64| <=
 max(3, 1e-4 count_fp64); VaR99 within 1e-4 relative of the FP64 order
 statistic."""
@@ -143,9 +145,11 @@ def test_full_size_path_ranges_add_up(mcr, wl, precision):
     and the S histogram bin by bin -- the per-path results do not depend on
     how the paths are grouped, which a run split over GPUs or resumed from a
     saved tally relies on.  The large n = 600 workload (1e7 paths) is checked
-    the same way with bf16x3."""
+    the same way with bf16x3, the n = 1024 one (1e6 paths, column-chunk
+    kernel) with fp32."""
     prec = mcr.PREC_FP32 if precision == "fp32" else mcr.PREC_BF16X3
-    for C in (wl.FUSED_N50,) + ((wl.LARGE_N600,) if precision == "bf16x3" else ()):
+    extra = (wl.LARGE_N600,) if precision == "bf16x3" else (wl.LARGEST_N1024,)
+    for C in (wl.FUSED_N50,) + extra:
         mu, sigma = wl.model_of(mcr, C)
         M = C["paths"]
         whole = mcr.run(mu, sigma, C["x"], C["k"], M, C["path_seed"], precision=prec,
@@ -157,6 +161,21 @@ def test_full_size_path_ranges_add_up(mcr, wl, precision):
         assert whole.count == sum(r.count for r in parts)
         assert np.array_equal(whole.s_hist, np.sum([r.s_hist for r in parts], axis=0))
         assert int(whole.s_hist.sum()) == M
+
+
+def test_largest_n1024_within_bar_of_fp64(mcr, orc, wl):
+    """The n = 1024 workload asked for at bf16x3 runs the column-chunk fp32
+    kernel; its count and VaR99 are within the bf16x3 tolerances of the FP64
+    oracle on the identical stream (over the paths the oracle finishes in
+    about a second)."""
+    C = wl.LARGEST_N1024
+    mu, sigma = wl.model_of(mcr, C)
+    p = {"mu": mu, "sigma": sigma, "x": C["x"], "k": C["k"], "paths": C["paths"],
+         "seed": C["path_seed"], "weights": np.full(C["n"], 1.0 / C["n"]), "alphas": C["alphas"]}
+    r = alone(mcr, p)
+    assert r.kernel == "valu_chunk"
+    P = anchor_fp64(mcr, orc, p, r)
+    assert P >= 10_000
 
 
 def test_near_flag_on_bf16x3_is_refused_and_auto_uses_fp32(mcr, wl):

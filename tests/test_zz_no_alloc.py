@@ -86,3 +86,15 @@ def test_noalloc_errors_match_run(mcr):
                 fn(mu, sigma, 0.02, 2, args["paths"], 1, precision=args["precision"],
                    near=args["near"])
             assert e.value.code == code
+
+
+def test_noalloc_refuses_more_than_608_assets(mcr):
+    """Past 608 assets a query runs through mcr_run only (its buffers are
+    sized past the static array's design); the allocation-free entry point
+    refuses it before touching the device."""
+    mcr.init(0)
+    mu, sigma = factor_like(640, 8, 101, T=1500)
+    with pytest.raises(mcr.McrError) as e:
+        mcr.run_noalloc(mu, sigma, 0.02, 64, 1000, 1)
+    assert e.value.code == mcr.MCR_E_UNSUPPORTED
+    assert mcr.run(mu, sigma, 0.02, 64, 1000, 1).kernel == "valu_chunk"
