@@ -13,7 +13,8 @@ export MCR_LIB_PATH="$ROOT/gpu-montecarlo-risk_amd/_lib/sanitize/libmcr_hip.so"
 export ORACLE_LIB_PATH="$ROOT/oracle/_build/sanitize/liboracle.so"
 export ASAN_OPTIONS="detect_leaks=0:halt_on_error=1:abort_on_error=1"
 export UBSAN_OPTIONS="halt_on_error=1:print_stacktrace=1"
-export LD_PRELOAD="$(gcc -print-file-name=libasan.so):$(gcc -print-file-name=libubsan.so)"
+# the sanitizer runtimes first (ASan must be), any preload already set kept after them
+export LD_PRELOAD="$(gcc -print-file-name=libasan.so):$(gcc -print-file-name=libubsan.so)${LD_PRELOAD:+:$LD_PRELOAD}"
 cd "$ROOT"
 python -m pytest -q -p no:cacheprovider -m "not gpu" tests/test_io.py tests/test_io_properties.py \
   tests/test_abi.py tests/test_oracle_rng.py tests/test_oracle_paths.py "$@"
