@@ -589,7 +589,7 @@ __global__ __launch_bounds__(64 * NW) void k_valu_tile(PathArgs a) {
 constexpr int kChunkBlocks = 128;  // 4-column blocks per chunk (even: a diagonal
                                    // block's two column blocks share a chunk)
 constexpr int kChunkNW = 16;
-constexpr int kChunkMB = (kChunkMaxN / kTileRows + 1 + kChunkNW - 1) / kChunkNW;  // blocks per wave
+constexpr int kChunkMB = (kChunkMaxN / kTileRows + kChunkNW - 1) / kChunkNW;  // row blocks per wave
 
 __global__ __launch_bounds__(64 * kChunkNW) void k_valu_chunk(PathArgs a) {
   constexpr int NW = kChunkNW;
@@ -599,10 +599,12 @@ __global__ __launch_bounds__(64 * kChunkNW) void k_valu_chunk(PathArgs a) {
   __shared__ unsigned sh_hist[kChunkMaxN + 1];
   const int n = a.n;
   const int nb = (n + 3) >> 2;
-  const int nrb = (n + kTileRows - 1) / kTileRows;  // row blocks; block nrb is the loss row
+  const int nrb = (n + kTileRows - 1) / kTileRows;  // row blocks
   const int ldl = nrb * kTileRows;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  // the loss row goes to the wave the snake deal would give block nrb
+  const bool loss_wave = ((nrb / NW) & 1 ? NW - 1 - nrb % NW : nrb % NW) == wave;
   const bool want_hist = a.s_hist != nullptr;
   if (want_hist) {
     for (int i = threadIdx.x; i <= n; i += blockDim.x) sh_hist[i] = 0u;
@@ -627,12 +629,17 @@ __global__ __launch_bounds__(64 * kChunkNW) void k_valu_chunk(PathArgs a) {
 #pragma unroll
       for (int r = 0; r < kTileRows; ++r) {
         const int i = rb * kTileRows + r;
-        acc[k][r] = rb < nrb ? (i < n ? mu[i] : 0.0f) : (rb == nrb && r == 0 ? mp32 : 0.0f);
+        acc[k][r] = (rb < nrb && i < n) ? mu[i] : 0.0f;
       }
     }
+    float g = mp32;  // loss row: g = w^T mu + sum_j v_j z_j, j ascending
     for (int jb0 = 0; jb0 < nb; jb0 += kChunkBlocks) {
       const int jb1 = min(nb, jb0 + kChunkBlocks);
-      for (int b = jb0 + wave; b < jb1; b += NW) {
+      // a fresh copy of the wave index per chunk keeps the compiler from
+      // holding every block's addresses across the chunk loop (SGPR spills)
+      int wv = wave;
+      asm volatile("" : "+s"(wv));
+      for (int b = jb0 + wv; b < jb1; b += NW) {
         float z[4];
         normals4(key, path, (uint32_t)b, z);
         zs[(b - jb0) * 64 + lane] = make_float4(z[0], z[1], z[2], z[3]);
@@ -640,22 +647,8 @@ __global__ __launch_bounds__(64 * kChunkNW) void k_valu_chunk(PathArgs a) {
       __syncthreads();
 #pragma unroll
       for (int k = 0; k < kChunkMB; ++k) {
-        const int rb = k * NW + ((k & 1) ? NW - 1 - wave : wave);
-        if (rb > nrb) continue;
-        if (rb == nrb) {
-          // loss row: g = w^T mu + sum_j v_j z_j, j ascending
-          float g = acc[k][0];
-          for (int jb = jb0; jb < jb1; ++jb) {
-            const float4 z4 = zs[(jb - jb0) * 64 + lane];
-            const int rem = n - 4 * jb;  // >= 4 except in the last block
-            g = __builtin_fmaf(v[4 * jb + 0], z4.x, g);
-            if (rem > 1) g = __builtin_fmaf(v[4 * jb + 1], z4.y, g);
-            if (rem > 2) g = __builtin_fmaf(v[4 * jb + 2], z4.z, g);
-            if (rem > 3) g = __builtin_fmaf(v[4 * jb + 3], z4.w, g);
-          }
-          acc[k][0] = g;
-          continue;
-        }
+        const int rb = k * NW + ((k & 1) ? NW - 1 - wv : wv);
+        if (rb >= nrb) continue;
         const int i0 = rb * kTileRows;
         const int J = i0 / 4;  // column blocks left of the diagonal block
         MCR_CONSTP(float) Lb = L + l32_offset(ldl, i0, 0);
@@ -690,15 +683,25 @@ __global__ __launch_bounds__(64 * kChunkNW) void k_valu_chunk(PathArgs a) {
           }
         }
       }
+      if (loss_wave) {
+        for (int jb = jb0; jb < jb1; ++jb) {
+          const float4 z4 = zs[(jb - jb0) * 64 + lane];
+          const int rem = n - 4 * jb;  // >= 4 except in the last block
+          g = __builtin_fmaf(v[4 * jb + 0], z4.x, g);
+          if (rem > 1) g = __builtin_fmaf(v[4 * jb + 1], z4.y, g);
+          if (rem > 2) g = __builtin_fmaf(v[4 * jb + 2], z4.z, g);
+          if (rem > 3) g = __builtin_fmaf(v[4 * jb + 3], z4.w, g);
+        }
+      }
       __syncthreads();  // the chunk's normals are read; the next chunk may overwrite them
     }
 
+    if (loss_wave) sLoss[lane] = -g;
     int S = 0;
     bool near = false;
 #pragma unroll
     for (int k = 0; k < kChunkMB; ++k) {
       const int rb = k * NW + ((k & 1) ? NW - 1 - wave : wave);
-      if (rb == nrb) sLoss[lane] = -acc[k][0];
       if (rb >= nrb) continue;
 #pragma unroll
       for (int r = 0; r < kTileRows; ++r) {
