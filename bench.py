@@ -334,7 +334,7 @@ def large_leg(mcr, np, wl):
     res["tail_k"] = {"k": kt, "count": t.count, "p_hat": t.p_hat, "std_error": t.std_error,
                      "ci": [t.ci_lower, t.ci_upper], "kernel_ms": t.ms_kernel,
                      "count_equals_histogram_tail": t.count == int(h.s_hist[kt:].sum())}
-    # past 608 assets every precision runs the column-chunk fp32 kernel
+    # past 767 assets every precision runs the column-chunk fp32 kernel
     C = wl.LARGEST_N1024
     mu2, sigma2 = wl.model_of(mcr, C)
     runs = [mcr.run(mu2, sigma2, C["x"], C["k"], C["paths"], C["path_seed"],

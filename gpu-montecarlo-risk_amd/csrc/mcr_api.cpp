@@ -232,14 +232,16 @@ int choose_kernel(const mcr_query* q, Plan* p) {
       set_error("unknown precision %d", q->precision);
       return MCR_E_ARG;
     }
-    if (n > kMaxN) {
-      // the column-chunk fp32 kernel is the only one past the tile and MFMA
-      // kernels' 608 (fp32 error is no larger than the bf16x3 products')
-      k = MCR_KERNEL_VALU_CHUNK;
-    } else if (q->precision == MCR_PREC_FP32 || (q->precision == MCR_PREC_AUTO && near)) {
-      k = n <= kRegMaxN ? MCR_KERNEL_VALU_REG : MCR_KERNEL_VALU_TILE;
-    } else {
+    const bool fp32 = q->precision == MCR_PREC_FP32 || (q->precision == MCR_PREC_AUTO && near);
+    if (fp32) {
+      k = n <= kRegMaxN ? MCR_KERNEL_VALU_REG
+                        : (n <= kMaxN ? MCR_KERNEL_VALU_TILE : MCR_KERNEL_VALU_CHUNK);
+    } else if (n <= kMfmaLdsMaxN) {
       k = n <= 32 * kMfmaMaxT - 1 ? MCR_KERNEL_MFMA : MCR_KERNEL_MFMA_LDS;
+    } else {
+      // past the LDS MFMA kernel's 24 row tiles the column-chunk fp32 kernel
+      // runs every precision (fp32 error is no larger than the bf16x3 products')
+      k = MCR_KERNEL_VALU_CHUNK;
     }
   }
   if (k == MCR_KERNEL_VALU_REG && n > kRegMaxN) {
@@ -254,9 +256,10 @@ int choose_kernel(const mcr_query* q, Plan* p) {
     set_error("unknown kernel %d", k);
     return MCR_E_ARG;
   }
-  if ((k == MCR_KERNEL_VALU_TILE || k == MCR_KERNEL_MFMA_LDS) && n > kMaxN) {
+  if ((k == MCR_KERNEL_VALU_TILE && n > kMaxN) || (k == MCR_KERNEL_MFMA_LDS && n > kMfmaLdsMaxN)) {
     set_error("%s kernel supports n <= %d (n = %d); MCR_KERNEL_VALU_CHUNK takes n <= %d",
-              k == MCR_KERNEL_VALU_TILE ? "tile" : "LDS MFMA", kMaxN, n, kChunkMaxN);
+              k == MCR_KERNEL_VALU_TILE ? "tile" : "LDS MFMA",
+              k == MCR_KERNEL_VALU_TILE ? kMaxN : kMfmaLdsMaxN, n, kChunkMaxN);
     return MCR_E_UNSUPPORTED;
   }
   if (near && (k == MCR_KERNEL_MFMA || k == MCR_KERNEL_MFMA_LDS)) {

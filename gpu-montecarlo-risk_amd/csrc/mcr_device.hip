@@ -826,7 +826,7 @@ hipError_t launch_paths(int kernel, const PathArgs& a, int num_cus, hipStream_t 
   }
   if (kernel == 4) {
     PathKernel k = mfma_big_kernel(a.T);
-    if (!k || a.n > kMaxN) return hipErrorInvalidValue;
+    if (!k || a.n > kMfmaLdsMaxN) return hipErrorInvalidValue;
     const int block = 64 * big_waves(a.T);
     const uint64_t per_cu = (uint64_t)resident_blocks((const void*)k, block);
     const uint64_t blocks = min_u64((a.npaths + 63) / 64, (uint64_t)num_cus * per_cu);

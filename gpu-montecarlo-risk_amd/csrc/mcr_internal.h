@@ -9,6 +9,8 @@ namespace mcr {
 constexpr int kMaxAlpha = 4;
 constexpr int kMaxN = 608;      // tile kernel: 64 paths x n normals in LDS (152 KiB)
 constexpr int kChunkMaxN = 1024;  // column-chunk kernel and the factor: one thread per row
+constexpr int kMfmaLdsMaxN = 32 * 24 - 1;  // LDS MFMA kernel, single queries: 24 row
+                                           // tiles of 32 (assets + loss row)
 constexpr int kRegMaxN = 64;    // register kernel: one row accumulator per asset
 constexpr int kMfmaMaxT = 4;    // MFMA kernel: 32-row tiles (assets + loss row)
 constexpr int kMfmaNB = 32;     // MFMA kernel: Philox blocks per path (n <= 127)

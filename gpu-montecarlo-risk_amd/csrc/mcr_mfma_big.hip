@@ -1,4 +1,5 @@
-// mcr_mfma_big.hip -- bf16x3 MFMA path kernel for large n (up to 608).
+// mcr_mfma_big.hip -- bf16x3 MFMA path kernel for large n (single queries up
+// to 767, batches up to 608).
 //
 // The large-n form of the reference kernel's correlate step
 // (src/05-GPU/monte_carlos_cuda.cu:51-57): R = mu + L Z as a GEMM
@@ -484,7 +485,7 @@ __device__ __forceinline__ void mfma_spec_groups(const PathArgs& a, uint64_t g0,
 template <int NC, int TW, int P>
 __global__ __launch_bounds__(64 * (NC + kProducers)) void k_mfma_spec(PathArgs a) {
   __shared__ SpecShared<NC, P> sh;
-  __shared__ unsigned sh_hist[kMaxN + 1];
+  __shared__ unsigned sh_hist[kMfmaLdsMaxN + 1];  // the 3-tile form runs up to 24 row tiles
   const bool want_hist = a.s_hist != nullptr;
   if (want_hist) {
     for (int i = threadIdx.x; i <= a.n; i += blockDim.x) sh_hist[i] = 0u;

@@ -24,7 +24,7 @@ FUSED_N50 = {"name": "fused_n50_100M", "n": 50, "factors": 5, "days": 2000, "pat
 LARGE_N600 = {"name": "large_n600_10M", "n": 600, "factors": 20, "days": 2500,
               "paths": 10_000_000, "k": 10, "x": 0.02, "alphas": (0.99, 0.999),
               "data_seed": 1003, "path_seed": 20251203}
-# Past the tile and MFMA kernels' 608 assets: the column-chunk fp32 kernel.
+# Past the LDS MFMA kernel's 767 assets: the column-chunk fp32 kernel.
 LARGEST_N1024 = {"name": "largest_n1024_1M", "n": 1024, "factors": 20, "days": 4000,
                  "paths": 1_000_000, "k": 10, "x": 0.02, "alphas": (0.99,),
                  "data_seed": 1006, "path_seed": 20251204}

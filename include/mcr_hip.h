@@ -51,10 +51,11 @@ extern "C" {
 #define MCR_KERNEL_VALU_REG 1  /* one path per lane, rows in registers, n <= 64 */
 #define MCR_KERNEL_VALU_TILE 2 /* 64 paths per workgroup, normals in LDS, n <= 608 */
 #define MCR_KERNEL_MFMA 3      /* 64 paths per wave on 32x32x16 bf16 MFMA, n <= 127 */
-#define MCR_KERNEL_MFMA_LDS 4  /* 64 paths per workgroup, normals staged in LDS, n <= 608 */
+#define MCR_KERNEL_MFMA_LDS 4  /* 64 paths per workgroup, normals staged in LDS, n <= 767
+                                  (24 row tiles; n <= 608 in batches and the server) */
 #define MCR_KERNEL_VALU_CHUNK 5 /* fp32, 64 paths per workgroup, normals in 512-column LDS
-                                   chunks, any n <= 1024; the only kernel for n > 608, which
-                                   every precision then runs */
+                                   chunks, any n <= 1024: fp32 queries past 608, and every
+                                   precision past 767 */
 
 /* flags */
 #define MCR_F_VAR 0x1   /* portfolio-loss quantiles (VaR) and tail sums at alpha[] */

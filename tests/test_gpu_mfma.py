@@ -41,15 +41,17 @@ def test_mfma_vs_fp64_same_stream(mcr, orc, n):
 
 @pytest.mark.parametrize("n,M", [(5, 100_000), (50, 100_000), (128, 40_000), (200, 20_000),
                                  (333, 10_000), (450, 8_000), (511, 6_000), (512, 6_000),
-                                 (520, 6_000), (600, 6_000), (608, 5_000)])
+                                 (520, 6_000), (600, 6_000), (608, 5_000), (609, 5_000),
+                                 (700, 4_000), (767, 3_000)])
 def test_mfma_lds_vs_fp64_same_stream(mcr, orc, n, M):
     """Large-n MFMA kernel (normals staged in LDS), same tolerances.  The sizes
     cover every single-query workgroup shape (T = 1, 2, 5, 7, 11 row tiles
     with four waves; 15, 16 with eight multiplying waves of 2 tiles beside
-    four generating waves; 17, 19, 20 with eight of 3 tiles), the boundary
+    four generating waves; 17, 19, 20, 22, 24 with eight of 3 tiles), the boundary
     between the last two (n = 511: T = 16; n = 512: T = 17), an odd step count with
-    two-step phases (n = 520: S = 33, the last phase half full) and the
-    largest n.  The batch shapes are covered by the batch tests."""
+    two-step phases (n = 520: S = 33, the last phase half full), the batches'
+    largest n (608) and the single queries' (767: 24 tiles, every multiplying
+    wave holding three).  The batch shapes are covered by the batch tests."""
     seed, x = 91, 0.02
     k = max(1, n // 5)
     mu, sigma = factor_like(n, min(20, n), 70 + n, T=max(4000, 2 * n))
@@ -65,7 +67,8 @@ def test_mfma_lds_vs_fp64_same_stream(mcr, orc, n, M):
     assert abs(r.var[0] - v64) <= 1e-4 * abs(v64), (r.var[0], v64)
 
 
-@pytest.mark.parametrize("n,M,kernel", [(50, 300_000, 3), (50, 200_000, 4), (600, 50_000, 4)])
+@pytest.mark.parametrize("n,M,kernel", [(50, 300_000, 3), (50, 200_000, 4), (600, 50_000, 4),
+                                        (767, 20_000, 4)])
 def test_mfma_s_histogram_matches_per_path_counts(mcr, n, M, kernel):
     """The in-kernel (n+1)-bin histogram of S equals the histogram of the
     per-path S the same launch stores, and its tail above k is the count."""
@@ -127,3 +130,23 @@ def test_mfma_relative_error_bar(mcr, orc, n, M, kernel):
     assert rel <= 1e-4, (r.count, ref["count"])
     v64 = np.sort(ref["loss"])[rank_of(0.99, M) - 1]
     assert abs(r.var[0] - v64) <= 1e-4 * abs(v64)
+
+
+def test_kernel_choice_between_608_and_767_assets(mcr):
+    """609-767 assets: bf16x3 (and the automatic choice) run the LDS MFMA
+    kernel, fp32 (and the automatic choice when the near count is asked for)
+    the column-chunk kernel; from 768 every precision runs the chunk kernel and
+    naming the LDS MFMA kernel is refused."""
+    mu, sigma = factor_like(700, 10, 7000, T=2000)
+    b = mcr.run(mu, sigma, 0.02, 70, 5000, 3, precision=mcr.PREC_BF16X3)
+    a = mcr.run(mu, sigma, 0.02, 70, 5000, 3, precision=mcr.PREC_AUTO)
+    f = mcr.run(mu, sigma, 0.02, 70, 5000, 3, precision=mcr.PREC_FP32)
+    nr = mcr.run(mu, sigma, 0.02, 70, 5000, 3, precision=mcr.PREC_AUTO, near=True)
+    assert (b.kernel, a.kernel, f.kernel, nr.kernel) == ("mfma_lds", "mfma_lds", "valu_chunk",
+                                                          "valu_chunk")
+    assert b.count == a.count and f.count == nr.count
+    mu, sigma = factor_like(768, 10, 7001, T=2000)
+    assert mcr.run(mu, sigma, 0.02, 70, 1000, 3, precision=mcr.PREC_BF16X3).kernel == "valu_chunk"
+    with pytest.raises(mcr.McrError) as e:
+        mcr.run(mu, sigma, 0.02, 70, 1000, 3, kernel=mcr.KERNEL_MFMA_LDS)
+    assert e.value.code == mcr.MCR_E_UNSUPPORTED

@@ -133,7 +133,7 @@ def test_tile_kernel_exact(mcr, orc, n):
 
 @pytest.mark.parametrize("n", [1, 5, 9, 50, 129, 257, 600, 608, 609, 700, 1000, 1024])
 def test_chunk_kernel_exact(mcr, orc, n):
-    """The column-chunk kernel (any n <= 1024; the only one past 608): per-path
+    """The column-chunk kernel (any n <= 1024; fp32 past 608): per-path
     S and loss bits, count, near-threshold count and S histogram equal the
     oracle's.  n = 609-1024 covers one and two 512-column chunks, a last
     chunk of one 4-column block (n = 1024 - 3 .. 1024 ends on block 255) and
@@ -165,10 +165,11 @@ def test_chunk_and_tile_kernels_agree_at_scale(mcr):
     assert a.count == b.count and a.var == b.var and a.tail_sum_fixed == b.tail_sum_fixed
 
 
-def test_past_608_assets_every_precision_runs_the_chunk_kernel(mcr):
-    """Above 608 assets every precision (and the automatic choice) runs the
-    column-chunk kernel and gives the same result; naming the tile or LDS MFMA
-    kernel there is refused, as is n = 1025."""
+def test_past_767_assets_every_precision_runs_the_chunk_kernel(mcr):
+    """Above 767 assets (the LDS MFMA kernel's 24 row tiles) every precision
+    (and the automatic choice) runs the column-chunk kernel and gives the same
+    result; naming the tile or an MFMA kernel there is refused, as is
+    n = 1025."""
     mu, sigma = factor_like(800, 10, 8000, T=2000)
     rs = [mcr.run(mu, sigma, 0.02, 80, 5000, 3, precision=p, alphas=(0.99,))
           for p in (mcr.PREC_FP32, mcr.PREC_BF16X3, mcr.PREC_AUTO)]

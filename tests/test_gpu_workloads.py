@@ -18,7 +18,8 @@ Bars:
   * at full size, the count and S histogram of one query equal the sums over
     the same paths run as four ranges (fp32 and bf16x3 at n = 50, bf16x3 at
     n = 600, fp32 on the column-chunk kernel at n = 1024);
-  * the n = 1024 workload (column-chunk kernel, chosen for every precision)
+  * the n = 1024 workload (column-chunk kernel, chosen for every precision
+    past 767)
     within the bf16x3 tolerances of the FP64 oracle on the identical stream.
 bf16x3 tolerances (as in test_gpu_mfma.py): |count - count_fp64| <=
 max(3, 1e-4 count_fp64); VaR99 within 1e-4 relative of the FP64 order
