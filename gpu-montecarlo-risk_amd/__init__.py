@@ -35,7 +35,7 @@ HEADER_PATHS = [HEADER_PATH, os.path.join(os.path.dirname(_HERE), "include", "mc
 MCR_OK = 0
 MCR_E_ARG, MCR_E_UNSUPPORTED, MCR_E_DEVICE, MCR_E_NOMEM, MCR_E_FACTOR = -1, -2, -3, -4, -5
 MCR_MAX_ALPHA = 4
-MCR_MAX_N, MCR_MAX_N_BATCH = 1024, 608
+MCR_MAX_N, MCR_MAX_N_BATCH, MCR_MAX_N_NOALLOC = 1024, 767, 608
 PREC_FP32, PREC_BF16X3, PREC_AUTO = 0, 1, 2
 KERNEL_AUTO, KERNEL_VALU_REG, KERNEL_VALU_TILE, KERNEL_MFMA, KERNEL_MFMA_LDS = 0, 1, 2, 3, 4
 KERNEL_VALU_CHUNK = 5

@@ -216,7 +216,8 @@ size_t layout(const Plan& p, char* base, DevBuffers* b) {
   return c.off;
 }
 
-static_assert(MCR_MAX_N == kChunkMaxN && MCR_MAX_N_BATCH == kMaxN,
+static_assert(MCR_MAX_N == kChunkMaxN && MCR_MAX_N_BATCH == kBatchMaxN &&
+                  MCR_MAX_N_NOALLOC == kMaxN,
               "public size limits follow the kernels'");
 
 int choose_kernel(const mcr_query* q, Plan* p) {
@@ -1047,8 +1048,8 @@ int check_batch_query(const mcr_query* q, int i) {
               q->precision);
     return MCR_E_UNSUPPORTED;
   }
-  if (q->n > kMaxN) {
-    set_error("query %d: batches take n <= %d (n = %d)", i, kMaxN, q->n);
+  if (q->n > kBatchMaxN) {
+    set_error("query %d: batches take n <= %d (n = %d)", i, kBatchMaxN, q->n);
     return MCR_E_UNSUPPORTED;
   }
   if (q->S_out || q->loss_out || (q->flags & (MCR_F_SHIST | MCR_F_NEAR))) {
@@ -1263,7 +1264,7 @@ uint64_t batch_signature(const mcr_query* qs, const BatchPlan& bp, int grid, int
 int enqueue_batch(const BatchPlan& bp, char* dev, char* hout, int grid, int streams) {
   const FactorArgs* fa = (const FactorArgs*)(dev + bp.offFa);
   const PrepArgs* pa = (const PrepArgs*)(dev + bp.offPa);
-  HIP_TRY(launch_factor_batch(fa, bp.nq, g_ctx.stream));
+  HIP_TRY(launch_factor_batch(fa, bp.nq, bp.max_n, g_ctx.stream));
   HIP_TRY(launch_prep_batch(pa, bp.nq, bp.max_n, g_ctx.stream));
   MCR_TRY(launch_runs(bp.runs, (const PathArgs*)(dev + bp.offQa),
                       (const WorkItem*)(dev + bp.offItems), (unsigned*)(dev + bp.offNext), grid,

@@ -312,9 +312,10 @@ __global__ __launch_bounds__(1024) void k_factor(FactorArgs a) {
   factor_body(a, sh);
 }
 
-// One workgroup per query of a batch (n <= kMaxN).
+// One workgroup per query of a batch (n <= MAXN).
+template <int MAXN>
 __global__ __launch_bounds__(1024) void k_factor_batch(const FactorArgs* fas) {
-  __shared__ FactorShared<kMaxN> sh;
+  __shared__ FactorShared<MAXN> sh;
   const FactorArgs a = fas[blockIdx.x];
   factor_body(a, sh);
 }
@@ -328,8 +329,14 @@ hipError_t launch_factor(const FactorArgs& a, hipStream_t stream) {
   return hipGetLastError();
 }
 
-hipError_t launch_factor_batch(const FactorArgs* fas, int nq, hipStream_t stream) {
-  hipLaunchKernelGGL(k_factor_batch, dim3(nq), dim3(1024), 0, stream, fas);
+// max_n: the batch's largest query (a panel of 78 KiB up to kMaxN, 99 KiB
+// up to kBatchMaxN)
+hipError_t launch_factor_batch(const FactorArgs* fas, int nq, int max_n, hipStream_t stream) {
+  if (max_n < 1 || max_n > kBatchMaxN) return hipErrorInvalidValue;
+  if (max_n <= kMaxN)
+    hipLaunchKernelGGL(k_factor_batch<kMaxN>, dim3(nq), dim3(1024), 0, stream, fas);
+  else
+    hipLaunchKernelGGL(k_factor_batch<kBatchMaxN>, dim3(nq), dim3(1024), 0, stream, fas);
   return hipGetLastError();
 }
 

@@ -9,8 +9,9 @@ namespace mcr {
 constexpr int kMaxAlpha = 4;
 constexpr int kMaxN = 608;      // tile kernel: 64 paths x n normals in LDS (152 KiB)
 constexpr int kChunkMaxN = 1024;  // column-chunk kernel and the factor: one thread per row
-constexpr int kMfmaLdsMaxN = 32 * 24 - 1;  // LDS MFMA kernel, single queries: 24 row
-                                           // tiles of 32 (assets + loss row)
+constexpr int kMfmaLdsMaxN = 32 * 24 - 1;  // LDS MFMA kernels: 24 row tiles of 32
+                                           // (assets + loss row)
+constexpr int kBatchMaxN = kMfmaLdsMaxN;   // batches and the serving loop
 constexpr int kRegMaxN = 64;    // register kernel: one row accumulator per asset
 constexpr int kMfmaMaxT = 4;    // MFMA kernel: 32-row tiles (assets + loss row)
 constexpr int kMfmaNB = 32;     // MFMA kernel: Philox blocks per path (n <= 127)
@@ -115,7 +116,7 @@ struct PrepArgs {
 // Launchers (mcr_device.hip).  All asynchronous on `stream`.
 hipError_t launch_factor(const FactorArgs& a, hipStream_t stream);
 hipError_t launch_prep(const PrepArgs& a, hipStream_t stream);
-hipError_t launch_factor_batch(const FactorArgs* fas, int nq, hipStream_t stream);
+hipError_t launch_factor_batch(const FactorArgs* fas, int nq, int max_n, hipStream_t stream);
 hipError_t launch_prep_batch(const PrepArgs* pas, int nq, int max_n, hipStream_t stream);
 
 // Batched persistent execution: work item = a run of 64-path groups of one query.

@@ -236,8 +236,8 @@ int check_servable(const mcr_query* q) {
     set_error("served queries run the bf16x3 MFMA kernels (precision %d)", q->precision);
     return MCR_E_UNSUPPORTED;
   }
-  if (q->n > kMaxN) {
-    set_error("served queries take n <= %d (n = %d)", kMaxN, q->n);
+  if (q->n > kBatchMaxN) {
+    set_error("served queries take n <= %d (n = %d)", kBatchMaxN, q->n);
     return MCR_E_UNSUPPORTED;
   }
   if (q->S_out || q->loss_out || (q->flags & (MCR_F_SHIST | MCR_F_NEAR))) {
@@ -795,7 +795,7 @@ int mcr_server::enqueue_round(RoundRec& rr, const std::vector<Job*>& fresh,
   if (nf) {
     HIP_TRY(hipMemcpyAsync(dr + off_fa, fa, nf * sizeof(FactorArgs), hipMemcpyHostToDevice, s_setup));
     HIP_TRY(hipMemcpyAsync(dr + off_pa, pa, nf * sizeof(PrepArgs), hipMemcpyHostToDevice, s_setup));
-    HIP_TRY(launch_factor_batch((const FactorArgs*)(dr + off_fa), (int)nf, s_setup));
+    HIP_TRY(launch_factor_batch((const FactorArgs*)(dr + off_fa), (int)nf, max_n, s_setup));
     HIP_TRY(launch_prep_batch((const PrepArgs*)(dr + off_pa), (int)nf, max_n, s_setup));
   }
   if (!items.empty())

@@ -38,8 +38,9 @@ extern "C" {
 #define MCR_E_FACTOR -5      /* covariance not factorable (non-finite input) */
 
 #define MCR_MAX_ALPHA 4
-#define MCR_MAX_N 1024      /* largest asset count of a single query (mcr_run) */
-#define MCR_MAX_N_BATCH 608 /* largest in batches, the serving loop and mcr_run_noalloc */
+#define MCR_MAX_N 1024        /* largest asset count of a single query (mcr_run) */
+#define MCR_MAX_N_BATCH 767   /* largest in batches and the serving loop */
+#define MCR_MAX_N_NOALLOC 608 /* largest through mcr_run_noalloc */
 
 /* precision / kernel family */
 #define MCR_PREC_FP32 0   /* fused fp32 VALU kernels: bit-exact with the C mirror */
@@ -52,7 +53,7 @@ extern "C" {
 #define MCR_KERNEL_VALU_TILE 2 /* 64 paths per workgroup, normals in LDS, n <= 608 */
 #define MCR_KERNEL_MFMA 3      /* 64 paths per wave on 32x32x16 bf16 MFMA, n <= 127 */
 #define MCR_KERNEL_MFMA_LDS 4  /* 64 paths per workgroup, normals staged in LDS, n <= 767
-                                  (24 row tiles; n <= 608 in batches and the server) */
+                                  (24 row tiles) */
 #define MCR_KERNEL_VALU_CHUNK 5 /* fp32, 64 paths per workgroup, normals in 512-column LDS
                                    chunks, any n <= 1024: fp32 queries past 608, and every
                                    precision past 767 */
@@ -70,8 +71,9 @@ extern "C" {
 #define MCR_F_TEST_WIDE 0x200
 
 typedef struct mcr_query {
-  int n;                /* assets, 1 <= n <= MCR_MAX_N (MCR_MAX_N_BATCH in batches,
-                           the serving loop and mcr_run_noalloc) */
+  int n;                /* assets, 1 <= n <= MCR_MAX_N (MCR_MAX_N_BATCH in batches
+                           and the serving loop, MCR_MAX_N_NOALLOC in
+                           mcr_run_noalloc) */
   const double *mu;     /* [n] mean daily returns */
   const double *sigma;  /* [n * ld_sigma] covariance, row-major */
   int ld_sigma;         /* row stride of sigma (>= n) */
@@ -149,8 +151,8 @@ int mcr_run(const mcr_query *q, mcr_result *r);
  * 16 MiB array of static device memory that the library holds on the device
  * until the process ends, and the host side uses static buffers: no
  * hipMalloc, hipFree, pinned or heap allocation.  Needs mcr_init first;
- * fails with MCR_E_UNSUPPORTED when n > MCR_MAX_N_BATCH or the query's
- * buffers exceed the array (they do not for n <= MCR_MAX_N_BATCH; per-path
+ * fails with MCR_E_UNSUPPORTED when n > MCR_MAX_N_NOALLOC or the query's
+ * buffers exceed the array (they do not for n <= MCR_MAX_N_NOALLOC; per-path
  * outputs are staged 2^18 paths at a time).  Calls share the one array, so calls that overlap on one
  * device race: they are serialised with mcr_run by the library's lock. */
 int mcr_run_noalloc(const mcr_query *q, mcr_result *r);

@@ -54,8 +54,9 @@ def test_batch_equals_alone(mcr):
 def test_batch_equals_alone_at_class_boundaries(mcr):
     """Sizes on both sides of every kernel-class boundary (the register-operand
     MFMA kernel up to n = 127, the LDS row-tile ranges 5-8, 9-12, 13-16 and
-    17-24 tiles, the 608 limit) batch to the results of the queries alone."""
-    sizes = (127, 128, 255, 256, 383, 384, 511, 512, 607, 608)
+    17-24 tiles, the factor kernel's larger panel from 609, the 767 limit)
+    batch to the results of the queries alone."""
+    sizes = (127, 128, 255, 256, 383, 384, 511, 512, 607, 608, 609, 700, 767)
     ps = portfolios(11, sizes=sizes, paths=[3000 + 17 * i for i in range(len(sizes))])
     for use_graph in (False, True):
         res, o = mcr.run_batch(ps, use_graph=use_graph, chunk_paths=512)
@@ -188,8 +189,8 @@ def test_batch_rejects_fp32_queries(mcr):
     assert e.value.code == mcr.MCR_E_UNSUPPORTED
 
 
-def test_batch_rejects_more_than_608_assets(mcr):
-    ps = portfolios(8, sizes=(10, 609), paths=[100, 100])
+def test_batch_rejects_more_than_767_assets(mcr):
+    ps = portfolios(8, sizes=(10, 768), paths=[100, 100])
     with pytest.raises(mcr.McrError) as e:
         mcr.run_batch(ps)
     assert e.value.code == mcr.MCR_E_UNSUPPORTED

@@ -217,7 +217,7 @@ def test_concurrent_submitters_and_poll(mcr):
 def test_rejects_unservable_queries(mcr):
     p = stream(9, 1, sizes=(10,))[0]
     with mcr.Server() as srv:
-        big = {"mu": np.zeros(700), "sigma": np.eye(700) * 1e-4, "weights": None}
+        big = {"mu": np.zeros(768), "sigma": np.eye(768) * 1e-4, "weights": None}
         for bad in ({"precision": mcr.PREC_FP32}, {"extra_flags": mcr.F_NEAR}, big):
             q = dict(p, **bad)
             with pytest.raises(mcr.McrError) as e:
@@ -239,6 +239,17 @@ def test_age_limit_changes_order_not_results(mcr):
         with pytest.raises(mcr.McrError) as e:
             mcr.Server(age_limit_ms=bad)
         assert e.value.code == mcr.MCR_E_ARG
+
+
+def test_queries_past_608_assets_equal_alone(mcr):
+    """Queries of 609-767 assets (22 and 24 row tiles; the factor kernel's
+    larger panel) served beside small ones equal the queries alone."""
+    ps = stream(12, 8, sizes=(20, 609, 700, 767), reuse=0.2, paths=(2_000, 30_000))
+    out, st = serve(mcr, ps, gaps_ms=0.5)
+    assert st.completed == len(ps) and st.failed == 0
+    assert max(len(p["mu"]) for p in ps) > 608
+    for p, (r, _) in zip(ps, out):
+        same(r, alone(mcr, p))
 
 
 def test_forked_round_launches_equal_one_stream(mcr):
