@@ -24,6 +24,10 @@ FUSED_N50 = {"name": "fused_n50_100M", "n": 50, "factors": 5, "days": 2000, "pat
 LARGE_N600 = {"name": "large_n600_10M", "n": 600, "factors": 20, "days": 2500,
               "paths": 10_000_000, "k": 10, "x": 0.02, "alphas": (0.99, 0.999),
               "data_seed": 1003, "path_seed": 20251203}
+# The LDS MFMA kernel's largest single query: 24 row tiles of 32.
+WIDEST_MFMA_N767 = {"name": "widest_mfma_n767_1M", "n": 767, "factors": 20, "days": 4000,
+                    "paths": 1_000_000, "k": 10, "x": 0.02, "alphas": (0.99,),
+                    "data_seed": 1007, "path_seed": 20251205}
 # Past the LDS MFMA kernel's 767 assets: the column-chunk fp32 kernel.
 LARGEST_N1024 = {"name": "largest_n1024_1M", "n": 1024, "factors": 20, "days": 4000,
                  "paths": 1_000_000, "k": 10, "x": 0.02, "alphas": (0.99,),
@@ -45,7 +49,7 @@ P99_BOUND_MS = 50.0
 
 def model_of(mcr, cfg):
     """(mu, sigma) of a single-query workload (PLAIN_N20, FUSED_N50, LARGE_N600,
-    LARGEST_N1024)."""
+    WIDEST_MFMA_N767, LARGEST_N1024)."""
     return mcr.synth_factor_model(cfg["n"], cfg["factors"], cfg["days"], cfg["data_seed"])
 
 

@@ -18,8 +18,8 @@ Bars:
   * at full size, the count and S histogram of one query equal the sums over
     the same paths run as four ranges (fp32 and bf16x3 at n = 50, bf16x3 at
     n = 600, fp32 on the column-chunk kernel at n = 1024);
-  * the n = 1024 workload (column-chunk kernel, chosen for every precision
-    past 767)
+  * the n = 767 workload (LDS MFMA kernel, 24 row tiles) and the n = 1024
+    one (column-chunk kernel, chosen for every precision past 767)
     within the bf16x3 tolerances of the FP64 oracle on the identical stream.
 bf16x3 tolerances (as in test_gpu_mfma.py): |count - count_fp64| <=
 max(3, 1e-4 count_fp64); VaR99 within 1e-4 relative of the FP64 order
@@ -162,6 +162,20 @@ def test_full_size_path_ranges_add_up(mcr, wl, precision):
         assert whole.count == sum(r.count for r in parts)
         assert np.array_equal(whole.s_hist, np.sum([r.s_hist for r in parts], axis=0))
         assert int(whole.s_hist.sum()) == M
+
+
+def test_widest_mfma_n767_within_bar_of_fp64(mcr, orc, wl):
+    """The n = 767 workload (24 row tiles, the LDS MFMA kernel's largest
+    single query) at bf16x3: count and VaR99 within the bf16x3 tolerances of
+    the FP64 oracle on the identical stream."""
+    C = wl.WIDEST_MFMA_N767
+    mu, sigma = wl.model_of(mcr, C)
+    p = {"mu": mu, "sigma": sigma, "x": C["x"], "k": C["k"], "paths": C["paths"],
+         "seed": C["path_seed"], "weights": np.full(C["n"], 1.0 / C["n"]), "alphas": C["alphas"]}
+    r = alone(mcr, p)
+    assert r.kernel == "mfma_lds"
+    P = anchor_fp64(mcr, orc, p, r)
+    assert P >= 10_000
 
 
 def test_largest_n1024_within_bar_of_fp64(mcr, orc, wl):
