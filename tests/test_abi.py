@@ -111,9 +111,16 @@ def test_server_footprint_and_servable_checks(mcr):
     assert wide >= base + 65536 * 4 and wide - b >= (65536 - wcap) * 4 - 256
     small = dict(p, mu=np.zeros(5), sigma=np.eye(5), alphas=())
     assert mcr.server_footprint(small) < 16384
+    # served queries take up to MCR_MAX_N_BATCH assets (24 row tiles)
+    widest = dict(p, mu=np.zeros(767), sigma=np.eye(767), weights=None)
+    assert mcr.server_footprint(widest) > 2 * 767 * 767 * 8
+    over = dict(p, mu=np.zeros(768), sigma=np.eye(768), weights=None)
+    assert mcr.MCR_MAX_N_BATCH == 767
     for bad, code in (({"precision": mcr.PREC_FP32}, mcr.MCR_E_UNSUPPORTED),
                       ({"extra_flags": mcr.F_NEAR}, mcr.MCR_E_UNSUPPORTED),
-                      ({"paths": 0}, mcr.MCR_E_ARG)):
+                      ({"paths": 0}, mcr.MCR_E_ARG),
+                      ({"mu": over["mu"], "sigma": over["sigma"], "weights": None},
+                       mcr.MCR_E_UNSUPPORTED)):
         with pytest.raises(mcr.McrError) as e:
             mcr.server_footprint(dict(p, **bad))
         assert e.value.code == code
@@ -168,3 +175,24 @@ def test_ctypes_structs_match_the_c_header(mcr, tmp_path):
         assert ctypes.sizeof(py) == got[(cname, "size")], cname
         for fname, _ in py._fields_:
             assert getattr(py, fname).offset == got[(cname, fname)], (cname, fname)
+
+
+def test_python_constants_match_the_c_header(mcr):
+    """Every MCR_* integer constant the Python view defines (limits, error
+    codes, precisions, kernel ids, flags) equals the header's #define."""
+    import re
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    hdr = open(os.path.join(root, "include", "mcr_hip.h")).read()
+    defs = {m.group(1): int(m.group(2), 0)
+            for m in re.finditer(r"^#define (MCR_\w+) (-?(?:0x[0-9a-fA-F]+|\d+))\b", hdr, re.M)}
+    checked = 0
+    for name, value in defs.items():
+        py = name if hasattr(mcr, name) else name[len("MCR_"):]
+        if hasattr(mcr, py):
+            assert getattr(mcr, py) == value, (name, getattr(mcr, py), value)
+            checked += 1
+    for name in ("MCR_MAX_N", "MCR_MAX_N_BATCH", "MCR_MAX_N_NOALLOC", "MCR_KERNEL_VALU_CHUNK",
+                 "MCR_E_UNSUPPORTED", "MCR_PREC_AUTO", "MCR_F_NEAR"):
+        assert name in defs
+    assert checked >= 15
