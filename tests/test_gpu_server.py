@@ -8,6 +8,7 @@ paths are keyed by (seed, path id) and reduced into integer / fixed-point
 tallies, so the mix, the round boundaries and the cache cannot change them."""
 from __future__ import annotations
 
+import os
 import threading
 import time
 
@@ -248,6 +249,29 @@ def test_queries_past_608_assets_equal_alone(mcr):
     out, st = serve(mcr, ps, gaps_ms=0.5)
     assert st.completed == len(ps) and st.failed == 0
     assert max(len(p["mu"]) for p in ps) > 608
+    for p, (r, _) in zip(ps, out):
+        same(r, alone(mcr, p))
+
+
+# MCR_SERVER_RANDOM_CASES raises the number of randomized cases for a longer run
+@pytest.mark.parametrize("seed", range(21, 21 + int(os.environ.get("MCR_SERVER_RANDOM_CASES", "3"))))
+def test_randomized_queries_and_options_equal_alone(mcr, seed):
+    """Random server options (round length, forks, age limit, cache on or
+    off) and random queries (1-767 assets, 1-200k paths, 0-4 VaR levels,
+    narrow or wide windows, repeated covariances): every served result equals
+    the same query alone."""
+    rng = np.random.default_rng(seed)
+    opts = {"round_ms": float(rng.choice([0.05, 0.5, 4.0])), "streams": int(rng.integers(1, 3)),
+            "age_limit_ms": float(rng.choice([0.0, 0.01, 5.0])),
+            "cache_entries": int(rng.choice([0, -1, 2]))}
+    sizes = (1, 3, 17, 64, 127, 128, 300, 511, 608, 609, 700, 767)
+    ps = stream(100 + seed, 14, sizes=sizes, reuse=0.3, paths=(1, 200_000))
+    for p in ps:
+        na = int(rng.integers(0, 5))
+        p["alphas"] = tuple(sorted(rng.choice([0.5, 0.9, 0.95, 0.99, 0.999], na, replace=False)))
+        p["extra_flags"] = int(rng.choice([0, 0, mcr.F_TEST_NARROW, mcr.F_TEST_WIDE]))
+    out, st = serve(mcr, ps, gaps_ms=float(rng.choice([0.0, 0.3, 2.0])), seed=seed, **opts)
+    assert st.completed == len(ps) and st.failed == 0, opts
     for p, (r, _) in zip(ps, out):
         same(r, alone(mcr, p))
 
