@@ -9,6 +9,8 @@ paths are keyed by (seed, path id) and all reductions are integer or exact
 order statistics, so batching cannot change them."""
 from __future__ import annotations
 
+import os
+
 import numpy as np
 import pytest
 
@@ -179,6 +181,32 @@ def test_forked_class_launches_equal_one_stream(mcr, streams):
         assert o.streams_used == streams
         for x, y in zip(res, ref):
             same(x, y)
+
+
+# MCR_BATCH_RANDOM_CASES raises the number of randomized cases for a longer run
+@pytest.mark.parametrize("seed", range(31, 31 + int(os.environ.get("MCR_BATCH_RANDOM_CASES", "3"))))
+def test_randomized_batches_equal_alone(mcr, seed):
+    """Random batches (2-12 portfolios of 1-767 assets, 1-100k paths, 0-4 VaR
+    levels, narrow or wide windows) under random item sizes, orders, launch
+    streams, workgroup shapes and graph replay: every result equals the
+    query alone."""
+    rng = np.random.default_rng(seed)
+    count = int(rng.integers(2, 13))
+    sizes = tuple(int(v) for v in rng.choice([1, 9, 64, 127, 128, 256, 383, 450, 512, 608,
+                                               640, 767], count))
+    paths = [int(np.exp(rng.uniform(0.0, np.log(100_000)))) for _ in range(count)]
+    ps = portfolios(seed, sizes=sizes, paths=paths)
+    for p in ps:
+        na = int(rng.integers(0, 5))
+        p["alphas"] = tuple(sorted(rng.choice([0.5, 0.9, 0.95, 0.99, 0.999], na, replace=False)))
+        p["extra_flags"] = int(rng.choice([0, 0, mcr.F_TEST_NARROW, mcr.F_TEST_WIDE]))
+    kw = {"chunk_paths": int(rng.choice([64, 512, 2048, 65536])), "order": int(rng.integers(0, 2)),
+          "streams": int(rng.integers(1, 5)),
+          "shapes": tuple(int(v) for v in rng.integers(1, 3, 4))}
+    for use_graph in (False, True, True):
+        res, _ = mcr.run_batch(ps, use_graph=use_graph, **kw)
+        for p, r in zip(ps, res):
+            same(r, alone(mcr, p, extra_flags=p["extra_flags"]))
 
 
 def test_batch_rejects_fp32_queries(mcr):
