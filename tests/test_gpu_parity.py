@@ -184,6 +184,38 @@ def test_past_767_assets_every_precision_runs_the_chunk_kernel(mcr):
     assert e.value.code == mcr.MCR_E_UNSUPPORTED
 
 
+# MCR_PARITY_RANDOM_CASES raises the number of randomized cases for a longer run
+@pytest.mark.parametrize("case", range(int(os.environ.get("MCR_PARITY_RANDOM_CASES", "4"))))
+def test_randomized_fp32_queries_match_oracle(mcr, orc, case):
+    """Random single queries on the fp32 kernels (every kernel valid for the
+    drawn n in 1-1024: register, tile, column chunk), path counts, path
+    offsets, seeds, thresholds, k and weights: per-path S and loss bits,
+    count, near-threshold count and S histogram equal the oracle's."""
+    rng = np.random.default_rng(4242 + case)
+    n = int(rng.choice([int(rng.integers(1, 65)), int(rng.integers(65, 609)),
+                        int(rng.integers(609, 1025))]))
+    kernels = ([mcr.KERNEL_VALU_REG] if n <= 64 else []) + \
+        ([mcr.KERNEL_VALU_TILE] if n <= 608 else []) + [mcr.KERNEL_VALU_CHUNK]
+    kernel = int(rng.choice(kernels))
+    M = int(np.exp(rng.uniform(0.0, np.log(4e6 / max(n, 40)))))
+    seed = int(rng.integers(0, 2**63)) * 2 + int(rng.integers(0, 2))
+    path0 = int(rng.choice([0, int(rng.integers(0, 2**40)), 2**32 - M // 2]))
+    x = float(rng.uniform(-0.005, 0.04))
+    k = int(rng.integers(0, n + 2))
+    mu, sigma = factor_like(n, int(rng.integers(1, min(20, n) + 1)), 900 + case, T=max(1500, 2 * n))
+    w = None if rng.uniform() < 0.5 else rng.dirichlet(np.ones(n))
+    model, _ = device_model(mcr, orc, mu, sigma, x, w)
+    r = mcr.run(mu, sigma, x, k, M, seed, weights=w, kernel=kernel, want_S=True, want_loss=True,
+                near=True, want_hist=True, path_offset=path0)
+    ref = orc.simulate_f32(model, seed, path0, M, k, want_S=True, want_loss=True, want_hist=True,
+                           nthreads=8)
+    tag = (n, kernel, M, path0, k)
+    assert np.array_equal(r.S, ref["S"]), tag
+    assert np.array_equal(r.loss.view(np.uint32), ref["loss"].view(np.uint32)), tag
+    assert (r.count, r.near_threshold) == (ref["count"], ref["near"]), tag
+    assert np.array_equal(r.s_hist, ref["hist"]), tag
+
+
 @pytest.mark.parametrize("M", [1, 63, 64, 65, 257, 1000])
 @pytest.mark.parametrize("kernel", [1, 2, 5])
 def test_ragged_path_counts(mcr, orc, M, kernel):
