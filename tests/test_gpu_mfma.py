@@ -11,6 +11,7 @@ Tolerances (stated here, checked below):
 from __future__ import annotations
 
 import math
+import os
 
 import numpy as np
 import pytest
@@ -130,6 +131,37 @@ def test_mfma_relative_error_bar(mcr, orc, n, M, kernel):
     assert rel <= 1e-4, (r.count, ref["count"])
     v64 = np.sort(ref["loss"])[rank_of(0.99, M) - 1]
     assert abs(r.var[0] - v64) <= 1e-4 * abs(v64)
+
+
+# MCR_MFMA_RANDOM_CASES raises the number of randomized cases for a longer run
+@pytest.mark.parametrize("case", range(int(os.environ.get("MCR_MFMA_RANDOM_CASES", "4"))))
+def test_randomized_bf16x3_queries_within_bar_of_fp64(mcr, orc, case):
+    """Random bf16x3 queries (n = 1-767 on the register-operand or LDS MFMA
+    kernel, random path counts and offsets, seeds, thresholds, k, weights,
+    VaR levels) against the FP64 oracle on the identical stream, with the
+    tolerances above."""
+    rng = np.random.default_rng(5151 + case)
+    n = int(rng.choice([int(rng.integers(1, 128)), int(rng.integers(128, 768))]))
+    M = int(np.exp(rng.uniform(np.log(50), np.log(1e7 / max(n, 64)))))
+    seed = int(rng.integers(0, 2**63))
+    path0 = int(rng.choice([0, int(rng.integers(0, 2**40))]))
+    x = float(rng.uniform(0.0, 0.04))
+    k = int(rng.integers(1, n + 1))
+    alphas = tuple(sorted(rng.choice([0.9, 0.95, 0.99, 0.999], int(rng.integers(1, 3)),
+                                     replace=False)))
+    mu, sigma = factor_like(n, int(rng.integers(1, min(20, n) + 1)), 950 + case, T=max(1500, 2 * n))
+    w = rng.dirichlet(np.ones(n)) if rng.uniform() < 0.5 else np.full(n, 1.0 / n)
+    r = mcr.run(mu, sigma, x, k, M, seed, weights=w, precision=mcr.PREC_BF16X3, alphas=alphas,
+                path_offset=path0)
+    assert r.kernel == ("mfma" if n <= 127 else "mfma_lds")
+    L, *_ = mcr.factor(sigma)
+    ref = orc.simulate_f64(L, mu, w, x, seed, path0, M, k, want_loss=True, nthreads=8)
+    tag = (n, M, k, path0)
+    assert abs(r.count - ref["count"]) <= max(3, 1e-4 * ref["count"]), (tag, r.count, ref["count"])
+    srt = np.sort(ref["loss"])
+    for i, a in enumerate(alphas):
+        v64 = srt[rank_of(a, M) - 1]
+        assert abs(r.var[i] - v64) <= 1e-4 * abs(v64), (tag, a, r.var[i], v64)
 
 
 def test_kernel_choice_between_608_and_767_assets(mcr):
