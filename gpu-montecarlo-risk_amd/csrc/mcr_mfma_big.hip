@@ -1,5 +1,5 @@
-// mcr_mfma_big.hip -- bf16x3 MFMA path kernel for large n (single queries up
-// to 767, batches up to 608).
+// mcr_mfma_big.hip -- bf16x3 MFMA path kernels for large n (up to 767: 24 row
+// tiles of 32).
 //
 // The large-n form of the reference kernel's correlate step
 // (src/05-GPU/monte_carlos_cuda.cu:51-57): R = mu + L Z as a GEMM

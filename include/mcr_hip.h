@@ -191,7 +191,7 @@ typedef struct mcr_batch_opts {
   int streams;       /* concurrent streams the class launches are forked over (1..4);
                         0 = the tuned default */
   int shapes[4];     /* workgroup shape of the LDS MFMA kernel per row-tile range
-                        (n 128-255, 256-383, 384-511, 512-608): 0 = the tuned shape,
+                        (n 128-255, 256-383, 384-511, 512-767): 0 = the tuned shape,
                         s > 0 = shape s - 1 (all shapes give identical results) */
   int streams_used;  /* out */
   int big_shapes[4]; /* out: shape used per range (0 = the measured default) */
